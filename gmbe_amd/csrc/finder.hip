@@ -1,0 +1,549 @@
+// finder.hip — persistent enumeration kernel and the plain finder behind the
+// C ABI (gmbe_finder_*).  Replaces IterFinderGpu6's constructor / Execute /
+// destructor (src/IterFinderGpu.cu:1056-1119) and the kernel launch
+// IterFinderKernel_6<<<MAX_BLOCKS, 512>>> (src/IterFinderGpu.cu:1104-1107).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "device_graph.hpp"
+#include "graph.hpp"
+#include "mbe_kernel.hpp"
+
+using namespace gmbe;
+
+namespace mbe {
+
+// Persistent grid: every wave loops over tasks until none are left.
+template <int WAVES, int LDS_PER_WAVE, int WMAX>
+__global__ __launch_bounds__(WAVES * 64) void mbe_persistent(Params P) {
+  __shared__ __attribute__((aligned(16))) char lds[WAVES * (LDS_PER_WAVE > 0 ? LDS_PER_WAVE : 16)];
+  __shared__ uint32_t s_off[WAVES][kWave + 1];
+  __shared__ int64_t s_start[WAVES][kWave];
+  const uint32_t wv = threadIdx.x / kWave;
+  const uint64_t gw = (uint64_t)blockIdx.x * WAVES + wv;
+  WaveMem m = wave_mem(P, gw, lds + wv * (LDS_PER_WAVE > 0 ? LDS_PER_WAVE : 16), LDS_PER_WAVE,
+                       s_off[wv], s_start[wv]);
+  Counters c;
+  Task t;
+  int k = 0;
+  uint32_t word = 0;
+  for (;;) {
+    int kind = next_work(P, word, t, k);
+    if (kind == 0) break;
+    if (kind == 2) {
+      run_task<WMAX>(P, m, t, k, c);
+    } else {
+      // an entry: children prefix + [c_i] of one split task
+      const uint32_t hdr = read_lane(word, 0);
+      const int kp = (int)(hdr & 0xffu);
+      const uint32_t cnt = hdr >> 8;
+      for (int i = 0; i < kMaxPrefix; ++i) t.v[i] = i < kp ? read_lane(word, 1 + i) : kNone;
+      for (uint32_t i = 0; i < cnt; ++i) {
+        t.v[kp] = read_lane(word, 8 + i);
+        run_task<WMAX>(P, m, t, kp + 1, c);
+      }
+    }
+    work_done(P);
+  }
+  if (lane_id() == 0) {
+    unsigned long long* a = P.acc;
+    __hip_atomic_fetch_add(a + A_MAXIMAL, (unsigned long long)c.maximal, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(a + A_NONMAX, (unsigned long long)c.nonmax, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(a + A_HSUM, (unsigned long long)c.hsum, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_xor(a + A_HXOR, (unsigned long long)c.hxor, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(a + A_NODES, (unsigned long long)c.nodes, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(a + A_TASKS, (unsigned long long)c.tasks, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(a + A_SPLITS, (unsigned long long)c.splits, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(a + A_OVERFLOW, (unsigned long long)c.overflow, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(a + A_BALG, (unsigned long long)c.balg, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Reset the queue's slot sequence numbers (slot i starts free for position i).
+__global__ void init_queue(unsigned long long* seq, unsigned long long cap) {
+  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < cap;
+       i += (unsigned long long)gridDim.x * blockDim.x)
+    seq[i] = i;
+}
+
+}  // namespace mbe
+
+namespace gmbe {
+
+#define HIP_TRY(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return set_error(GMBE_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                       __FILE__, __LINE__);                                                  \
+  } while (0)
+
+typedef void (*KernelFn)(mbe::Params);
+
+// control counters live kCtlStride u64 (256 bytes) apart so that polling one
+// never contends with atomics on another
+constexpr int kCtlStride = 32;
+enum CtlSlot { C_ROOT_NEXT = 0, C_WORK_LEFT, C_QHEAD, C_QTAIL, C_LIST_CURSOR, C_DEFERRED, C_NSLOTS };
+
+struct KernelChoice {
+  KernelFn fn;
+  int waves;
+  int lds_per_wave;
+};
+
+static KernelChoice pick_kernel(int lds_kb, bool wide = false) {
+  using namespace mbe;
+  if (wide) return {mbe_persistent<4, 8192, kMaxW>, 4, 8192};
+  switch (lds_kb) {
+    case 0: return {mbe_persistent<4, 0, kMainW>, 4, 0};
+    case 4: return {mbe_persistent<4, 4096, kMainW>, 4, 4096};
+    case 16: return {mbe_persistent<4, 16384, kMainW>, 4, 16384};
+    case 24: return {mbe_persistent<4, 24576, kMainW>, 4, 24576};
+    default: return {mbe_persistent<4, 8192, kMainW>, 4, 8192};
+  }
+}
+
+int Engine::ensure_device(int dev) {
+  if (device >= 0) return GMBE_OK;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0)
+    return set_error(GMBE_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+  HIP_TRY(hipSetDevice(dev));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, dev));
+  cu_count = prop.multiProcessorCount;
+  device = dev;
+  return GMBE_OK;
+}
+
+int DeviceGraph::upload(const gmbe_graph& g) {
+  nU = g.nU;
+  nV = g.nV;
+  E = g.E;
+  swapped = g.swapped;
+  dmaxV = 0;
+  for (uint32_t v = 0; v < g.nV; ++v)
+    dmaxV = std::max<uint32_t>(dmaxV, (uint32_t)(g.v_off[v + 1] - g.v_off[v]));
+  std::vector<uint64_t> kU, kV;
+  hash_keys(g, kU, kV);
+  size_t b_voff = (size_t)(nV + 1) * 8, b_uoff = (size_t)(nU + 1) * 8;
+  size_t b_adj = (size_t)std::max<uint64_t>(E, 1) * 4;
+  size_t b_kU = (size_t)std::max<uint32_t>(nU, 1) * 8, b_kV = (size_t)std::max<uint32_t>(nV, 1) * 8;
+  HIP_TRY(hipMalloc(&v_off, b_voff));
+  HIP_TRY(hipMalloc(&u_off, b_uoff));
+  HIP_TRY(hipMalloc(&v_adj, b_adj));
+  HIP_TRY(hipMalloc(&u_adj, b_adj));
+  HIP_TRY(hipMalloc(&kUd, b_kU));
+  HIP_TRY(hipMalloc(&kVd, b_kV));
+  HIP_TRY(hipMemcpy(v_off, g.v_off.data(), b_voff, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(u_off, g.u_off.data(), b_uoff, hipMemcpyHostToDevice));
+  if (E) {
+    HIP_TRY(hipMemcpy(v_adj, g.v_adj.data(), (size_t)E * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(u_adj, g.u_adj.data(), (size_t)E * 4, hipMemcpyHostToDevice));
+  }
+  if (nU) HIP_TRY(hipMemcpy(kUd, kU.data(), (size_t)nU * 8, hipMemcpyHostToDevice));
+  if (nV) HIP_TRY(hipMemcpy(kVd, kV.data(), (size_t)nV * 8, hipMemcpyHostToDevice));
+  return GMBE_OK;
+}
+
+void DeviceGraph::release() {
+  (void)hipFree(v_off);
+  (void)hipFree(u_off);
+  (void)hipFree(v_adj);
+  (void)hipFree(u_adj);
+  (void)hipFree(kUd);
+  (void)hipFree(kVd);
+  v_off = u_off = nullptr;
+  v_adj = u_adj = nullptr;
+  kUd = kVd = nullptr;
+}
+
+mbe::DevGraph DeviceGraph::view() const {
+  mbe::DevGraph d;
+  d.v_off = v_off;
+  d.v_adj = v_adj;
+  d.u_off = u_off;
+  d.u_adj = u_adj;
+  d.kU = kUd;
+  d.kV = kVd;
+  d.nU = nU;
+  d.nV = nV;
+  d.dmaxV = dmaxV;
+  d.swapped = (uint32_t)swapped;
+  return d;
+}
+
+static uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+// Per-wave slab layout for a graph with nV enumeration-side vertices and
+// maximum enumeration-side degree dmaxV.
+mbe::Scratch scratch_layout(uint32_t nV, uint32_t dmaxV, uint64_t arena_bytes) {
+  mbe::Scratch s{};
+  uint64_t o = 0;
+  auto take = [&](uint64_t bytes) {
+    uint64_t r = o;
+    o = align_up(o + bytes, 256);
+    return r;
+  };
+  uint64_t nv = std::max<uint32_t>(nV, 1), d = (uint64_t)dmaxV + 2;
+  s.nW0 = (uint32_t)((nv + 63) / 64);
+  s.nW1 = (uint32_t)((s.nW0 + 63) / 64);
+  s.o_cmap = take(nv * 4);
+  s.o_bm0 = take((uint64_t)s.nW0 * 8);
+  s.o_bm1 = take((uint64_t)s.nW1 * 8);
+  s.o_L = take(d * 4);
+  s.o_Lp = take(d * 4);
+  s.o_T = take(d * 4);
+  s.o_kL = take(d * 8);
+  s.o_U = take(nv * 4);
+  s.o_Ucnt = take(nv * 4);
+  s.o_C = take(nv * 4);
+  s.o_kC = take(nv * 8);
+  s.o_R = take(nv * 4);
+  s.o_fr = take((d + 2) * sizeof(mbe::Frame));
+  s.o_arena = take(arena_bytes);
+  s.arena_bytes = arena_bytes;
+  s.slab = o;
+  return s;
+}
+
+}  // namespace gmbe
+
+// ---------------------------------------------------------------------------
+
+struct gmbe_finder {
+  const gmbe_graph* g = nullptr;
+  int sel = 2, bound_height = 20, bound_size = 1500;
+  // knobs
+  int64_t waves_per_block = 4, blocks = 0, prune = 1, max_prefix = 4, lds_kb = 8;
+  int64_t queue_log2 = 22, list_cap = 0, arena_kb = 4096, root_stride = 1, root_phase = 0;
+  gmbe::Engine eng;
+  gmbe::DeviceGraph dg;
+  // device buffers
+  char* scratch = nullptr;
+  uint64_t scratch_bytes = 0;
+  mbe::Scratch layout{};
+  int nwaves = 0, nblocks = 0;
+  uint32_t* q = nullptr;
+  unsigned long long* qseq = nullptr;
+  uint64_t qcap = 0;
+  // control counters, each on its own 256-byte line (kCtlStride u64 apart):
+  // root_next, work_left, qhead, qtail, list_cursor, n_deferred
+  unsigned long long* ctl = nullptr;
+  unsigned long long* acc = nullptr;
+  uint32_t* list = nullptr;
+  uint64_t list_words = 0;
+  mbe::Task* deferred = nullptr;
+  uint64_t deferred_cap = 1ull << 20;
+  uint64_t last_deferred = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool ran = false;
+  uint64_t list_used = 0, list_ovf = 0;
+};
+
+using namespace gmbe;
+
+static int finder_alloc(gmbe_finder* f) {
+  KernelChoice kc = pick_kernel((int)f->lds_kb);
+  int per_sm = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_sm, (const void*)kc.fn, kc.waves * 64,
+                                                       0));
+  if (per_sm < 1) per_sm = 1;
+  int blocks = f->blocks > 0 ? (int)f->blocks : per_sm * f->eng.cu_count;
+  int nwaves = blocks * kc.waves;
+  mbe::Scratch lay =
+      scratch_layout(f->dg.nV, f->dg.dmaxV, (uint64_t)f->arena_kb * 1024ull);
+  uint64_t bytes = lay.slab * (uint64_t)nwaves;
+  if (bytes != f->scratch_bytes || nwaves != f->nwaves) {
+    if (f->scratch) (void)hipFree(f->scratch);
+    f->scratch = nullptr;
+    f->scratch_bytes = 0;
+    hipError_t e = hipMalloc(&f->scratch, bytes);
+    if (e != hipSuccess)
+      return set_error(GMBE_ERR_NOMEM, "scratch of %llu bytes (%d waves): %s",
+                       (unsigned long long)bytes, nwaves, hipGetErrorString(e));
+    HIP_TRY(hipMemsetAsync(f->scratch, 0, bytes, f->stream));
+    f->scratch_bytes = bytes;
+  }
+  lay.base = f->scratch;
+  f->layout = lay;
+  f->nwaves = nwaves;
+  f->nblocks = blocks;
+  uint64_t qcap = 1ull << f->queue_log2;
+  if (qcap != f->qcap) {
+    if (f->q) (void)hipFree(f->q);
+    if (f->qseq) (void)hipFree(f->qseq);
+    f->q = nullptr;
+    f->qseq = nullptr;
+    HIP_TRY(hipMalloc(&f->q, qcap * mbe::kEntryWords * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc(&f->qseq, qcap * 8));
+    f->qcap = qcap;
+  }
+  if (!f->deferred) HIP_TRY(hipMalloc(&f->deferred, f->deferred_cap * sizeof(mbe::Task)));
+  if ((uint64_t)f->list_cap != f->list_words) {
+    if (f->list) (void)hipFree(f->list);
+    f->list = nullptr;
+    if (f->list_cap > 0) HIP_TRY(hipMalloc(&f->list, (size_t)f->list_cap * 4));
+    f->list_words = (uint64_t)f->list_cap;
+  }
+  return GMBE_OK;
+}
+
+extern "C" {
+
+int gmbe_finder_create(const gmbe_graph* g, int sel, int bound_height, int bound_size, int ngpus,
+                       gmbe_finder** out) {
+  if (!g || !out) return set_error(GMBE_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (sel < 0 || sel > 3) return set_error(GMBE_ERR_ARG, "sel must be 0..3 (got %d)", sel);
+  if (ngpus != 1 || sel == 3)
+    return set_error(GMBE_ERR_UNSUPPORTED,
+                     "this build runs on one GPU (sel=%d ngpus=%d); see the sharding design",
+                     sel, ngpus);
+  if (bound_height < 0 || bound_size < 0)
+    return set_error(GMBE_ERR_ARG, "bounds must be non-negative");
+  gmbe_finder* f = new gmbe_finder();
+  f->g = g;
+  f->sel = sel;
+  f->bound_height = bound_height;
+  f->bound_size = bound_size;
+  int rc = f->eng.ensure_device(0);
+  if (rc) {
+    delete f;
+    return rc;
+  }
+  if (hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&f->ev0) != hipSuccess || hipEventCreate(&f->ev1) != hipSuccess) {
+    delete f;
+    return set_error(GMBE_ERR_HIP, "stream/event creation failed");
+  }
+  rc = f->dg.upload(*g);
+  if (rc == GMBE_OK && hipMalloc(&f->ctl, 8 * kCtlStride * sizeof(unsigned long long)) != hipSuccess)
+    rc = set_error(GMBE_ERR_NOMEM, "control block");
+  if (rc == GMBE_OK && hipMalloc(&f->acc, mbe::A_COUNT * sizeof(unsigned long long)) != hipSuccess)
+    rc = set_error(GMBE_ERR_NOMEM, "result block");
+  if (rc) {
+    gmbe_finder_destroy(f);
+    return rc;
+  }
+  *out = f;
+  return GMBE_OK;
+}
+
+int gmbe_finder_set(gmbe_finder* f, const char* key, int64_t value) {
+  if (!f || !key) return set_error(GMBE_ERR_ARG, "null argument");
+  std::string k(key);
+  auto range = [&](int64_t lo, int64_t hi) -> int {
+    if (value < lo || value > hi)
+      return set_error(GMBE_ERR_ARG, "%s=%lld out of range [%lld, %lld]", key, (long long)value,
+                       (long long)lo, (long long)hi);
+    return GMBE_OK;
+  };
+  int rc;
+  if (k == "waves_per_block") { if ((rc = range(4, 4))) return rc; f->waves_per_block = value; }
+  else if (k == "blocks") { if ((rc = range(0, 1 << 20))) return rc; f->blocks = value; }
+  else if (k == "prune") { if ((rc = range(0, 1))) return rc; f->prune = value; }
+  else if (k == "max_prefix") { if ((rc = range(2, mbe::kMaxPrefix))) return rc; f->max_prefix = value; }
+  else if (k == "lds_kb") {
+    if (value != 0 && value != 4 && value != 8 && value != 16 && value != 24)
+      return set_error(GMBE_ERR_ARG, "lds_kb must be 0, 4, 8, 16 or 24");
+    f->lds_kb = value;
+  }
+  else if (k == "queue_log2") { if ((rc = range(4, 30))) return rc; f->queue_log2 = value; }
+  else if (k == "list_cap") { if ((rc = range(0, (int64_t)1 << 34))) return rc; f->list_cap = value; }
+  else if (k == "arena_kb") { if ((rc = range(64, 1 << 20))) return rc; f->arena_kb = value; }
+  else if (k == "root_stride") { if ((rc = range(1, 1 << 30))) return rc; f->root_stride = value; }
+  else if (k == "root_phase") { if ((rc = range(0, (1 << 30) - 1))) return rc; f->root_phase = value; }
+  else if (k == "bound_height") { if ((rc = range(0, 1 << 30))) return rc; f->bound_height = (int)value; }
+  else if (k == "bound_size") { if ((rc = range(0, (int64_t)1 << 30))) return rc; f->bound_size = (int)value; }
+  else return set_error(GMBE_ERR_ARG, "unknown knob '%s'", key);
+  return GMBE_OK;
+}
+
+int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res) {
+  if (!f || !res) return set_error(GMBE_ERR_ARG, "null argument");
+  if (f->root_phase >= f->root_stride)
+    return set_error(GMBE_ERR_ARG, "root_phase must be < root_stride");
+  memset(res, 0, sizeof(*res));
+  HIP_TRY(hipSetDevice(f->eng.device));
+  int rc = finder_alloc(f);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(f->stream));
+  double t0 = now_s();  // start_time_ (src/IterFinderGpu.cu:1096)
+  uint64_t nV = f->dg.nV, stride = (uint64_t)f->root_stride, phase = (uint64_t)f->root_phase;
+  uint64_t n_roots = nV > phase ? (nV - phase + stride - 1) / stride : 0;
+  std::vector<unsigned long long> ctl_h(8 * kCtlStride, 0ull);
+  ctl_h[C_WORK_LEFT * kCtlStride] = n_roots;
+  HIP_TRY(hipMemcpyAsync(f->ctl, ctl_h.data(), ctl_h.size() * 8, hipMemcpyHostToDevice,
+                         f->stream));
+  HIP_TRY(hipMemsetAsync(f->acc, 0, mbe::A_COUNT * 8, f->stream));
+  hipLaunchKernelGGL(mbe::init_queue, dim3(256), dim3(256), 0, f->stream, f->qseq,
+                     (unsigned long long)f->qcap);
+  HIP_TRY(hipGetLastError());
+  mbe::Params P{};
+  P.g = f->dg.view();
+  P.s.root_next = f->ctl + C_ROOT_NEXT * kCtlStride;
+  P.s.work_left = f->ctl + C_WORK_LEFT * kCtlStride;
+  P.s.qhead = f->ctl + C_QHEAD * kCtlStride;
+  P.s.qtail = f->ctl + C_QTAIL * kCtlStride;
+  P.list_cursor = f->ctl + C_LIST_CURSOR * kCtlStride;
+  P.s.n_deferred = f->ctl + C_DEFERRED * kCtlStride;
+  P.s.deferred = f->deferred;
+  P.s.deferred_cap = f->deferred_cap;
+  P.s.q = f->q;
+  P.s.qseq = f->qseq;
+  P.s.qcap = f->qcap;
+  P.s.n_roots = n_roots;
+  P.s.stride = (uint32_t)stride;
+  P.s.phase = (uint32_t)phase;
+  P.s.root_tasks = nullptr;
+  P.w = f->layout;
+  P.acc = f->acc;
+  P.list = f->list;
+  P.list_cap = f->list_words;
+  P.bound_height = f->bound_height;
+  P.bound_size = f->bound_size;
+  P.prune = (int)f->prune;
+  P.max_prefix = (int)f->max_prefix;
+  P.split = f->sel == 2;
+  KernelChoice kc = pick_kernel((int)f->lds_kb);
+  HIP_TRY(hipEventRecord(f->ev0, f->stream));
+  if (n_roots > 0) {
+    hipLaunchKernelGGL(kc.fn, dim3(f->nblocks), dim3(kc.waves * 64), 0, f->stream, P);
+    HIP_TRY(hipGetLastError());
+  }
+  std::vector<unsigned long long> ctl_out(8 * kCtlStride, 0ull);
+  HIP_TRY(hipMemcpyAsync(ctl_out.data(), f->ctl, ctl_out.size() * 8, hipMemcpyDeviceToHost,
+                         f->stream));
+  HIP_TRY(hipStreamSynchronize(f->stream));
+  f->last_deferred = std::min<uint64_t>(ctl_out[C_DEFERRED * kCtlStride], f->deferred_cap);
+  if (f->last_deferred > 0) {
+    // wide-bitset pass over the deferred tasks (same queue, fresh counters)
+    std::vector<unsigned long long> ctl2(8 * kCtlStride, 0ull);
+    ctl2[C_WORK_LEFT * kCtlStride] = f->last_deferred;
+    ctl2[C_LIST_CURSOR * kCtlStride] = ctl_out[C_LIST_CURSOR * kCtlStride];
+    HIP_TRY(hipMemcpyAsync(f->ctl, ctl2.data(), ctl2.size() * 8, hipMemcpyHostToDevice,
+                           f->stream));
+    hipLaunchKernelGGL(mbe::init_queue, dim3(256), dim3(256), 0, f->stream, f->qseq,
+                       (unsigned long long)f->qcap);
+    P.s.root_tasks = f->deferred;
+    P.s.n_roots = f->last_deferred;
+    P.s.deferred = nullptr;
+    P.s.deferred_cap = 0;
+    KernelChoice wk = pick_kernel(8, true);
+    hipLaunchKernelGGL(wk.fn, dim3(f->nblocks), dim3(wk.waves * 64), 0, f->stream, P);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipEventRecord(f->ev1, f->stream));
+  unsigned long long acc_h[mbe::A_COUNT];
+  HIP_TRY(hipMemcpyAsync(acc_h, f->acc, sizeof(acc_h), hipMemcpyDeviceToHost, f->stream));
+  HIP_TRY(hipMemcpyAsync(ctl_out.data(), f->ctl, ctl_out.size() * 8, hipMemcpyDeviceToHost,
+                         f->stream));
+  HIP_TRY(hipStreamSynchronize(f->stream));
+  double t1 = now_s();
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, f->ev0, f->ev1));
+  res->maximal = acc_h[mbe::A_MAXIMAL];
+  res->non_maximal = acc_h[mbe::A_NONMAX];
+  res->hash_sum = acc_h[mbe::A_HSUM];
+  res->hash_xor = acc_h[mbe::A_HXOR];
+  res->nodes = acc_h[mbe::A_NODES];
+  res->tasks = acc_h[mbe::A_TASKS];
+  res->splits = acc_h[mbe::A_SPLITS];
+  res->overflow = acc_h[mbe::A_OVERFLOW];
+  res->alg_bytes = acc_h[mbe::A_BALG];
+  res->exe_time_s = t1 - t0;
+  res->kernel_time_s = ms * 1e-3;
+  f->ran = true;
+  f->list_used = std::min<uint64_t>(ctl_out[C_LIST_CURSOR * kCtlStride], f->list_words);
+  f->list_ovf = acc_h[mbe::A_LIST_OVF];
+  if (res->overflow)
+    return set_error(GMBE_ERR_CAPACITY,
+                     "%llu task(s) exceeded a per-wave capacity (arena_kb=%lld, |L|<=%d words "
+                     "or queue); the result is incomplete",
+                     (unsigned long long)res->overflow, (long long)f->arena_kb, mbe::kMaxW * 64);
+  return GMBE_OK;
+}
+
+int gmbe_finder_list(gmbe_finder* f, uint32_t* buf, size_t cap, size_t* used) {
+  if (!f || !used) return set_error(GMBE_ERR_ARG, "null argument");
+  if (!f->ran || !f->list) return set_error(GMBE_ERR_STATE, "no list-mode run (set list_cap)");
+  if (f->list_ovf)
+    return set_error(GMBE_ERR_CAPACITY, "device list buffer overflowed (list_cap=%lld)",
+                     (long long)f->list_cap);
+  std::vector<uint32_t> raw(f->list_used);
+  if (f->list_used)
+    HIP_TRY(hipMemcpy(raw.data(), f->list, f->list_used * 4, hipMemcpyDeviceToHost));
+  *used = f->list_used;
+  if (!buf) return GMBE_OK;
+  if (cap < f->list_used) return set_error(GMBE_ERR_CAPACITY, "buffer too small");
+  // internal (L, R) -> original (A, B)
+  const gmbe_graph* g = f->g;
+  size_t i = 0, o = 0;
+  while (i < raw.size()) {
+    uint32_t nL = raw[i], nR = raw[i + 1];
+    const uint32_t* L = &raw[i + 2];
+    const uint32_t* R = L + nL;
+    if (!g->swapped) {
+      buf[o++] = nL;
+      buf[o++] = nR;
+      for (uint32_t j = 0; j < nL; ++j) buf[o++] = g->u_orig[L[j]];
+      for (uint32_t j = 0; j < nR; ++j) buf[o++] = g->v_orig[R[j]];
+    } else {
+      buf[o++] = nR;
+      buf[o++] = nL;
+      for (uint32_t j = 0; j < nR; ++j) buf[o++] = g->v_orig[R[j]];
+      for (uint32_t j = 0; j < nL; ++j) buf[o++] = g->u_orig[L[j]];
+    }
+    i += 2 + (size_t)nL + nR;
+  }
+  return GMBE_OK;
+}
+
+void gmbe_finder_destroy(gmbe_finder* f) {
+  if (!f) return;
+  if (f->eng.device >= 0) (void)hipSetDevice(f->eng.device);
+  if (f->stream) (void)hipStreamSynchronize(f->stream);
+  f->dg.release();
+  (void)hipFree(f->scratch);
+  (void)hipFree(f->q);
+  (void)hipFree(f->qseq);
+  (void)hipFree(f->ctl);
+  (void)hipFree(f->acc);
+  (void)hipFree(f->list);
+  (void)hipFree(f->deferred);
+  if (f->ev0) (void)hipEventDestroy(f->ev0);
+  if (f->ev1) (void)hipEventDestroy(f->ev1);
+  if (f->stream) (void)hipStreamDestroy(f->stream);
+  delete f;
+}
+
+int gmbe_device_info(char* buf, size_t cap, int* cu_count) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0)
+    return set_error(GMBE_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, 0));
+  if (buf && cap) snprintf(buf, cap, "%s (%s)", prop.name, prop.gcnArchName);
+  if (cu_count) *cu_count = prop.multiProcessorCount;
+  return GMBE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,246 @@
+/*
+ * gmbe.h — C ABI of the MI355X maximal-biclique-enumeration engine.
+ *
+ * Every entry point takes plain pointers and sizes; no HIP or torch types
+ * cross this boundary.  Return codes: 0 = ok, negative = error (see
+ * GMBE_ERR_*); the message of the last error on the calling thread is
+ * returned by gmbe_last_error().  The library never calls exit().
+ *
+ * The reference's surface is a C++ class API used from src/main.cu:
+ *   Option{uvtrans, order, fast_mode}                 src/BiGraph.h:13-22
+ *   CSRBiGraph::Read / CopyToGpu / Reset              src/BiGraph.h:23-38, src/BiGraph.cu:13-278,413-454
+ *   BicliqueFinder::Execute / PrintResult             src/BicliqueFinder.h:18-30, src/BicliqueFinder.cu:9-16
+ *   IterFinderGpu6(graph, bound_height, bound_size)   src/IterFinderGpu.h:77-90, src/IterFinderGpu.cu:1056-1119
+ * Each declaration below names the reference interface it replaces.  The C++
+ * drop-in classes with the reference's names live in gmbe_amd/csrc/gmbe.hpp
+ * and are built on these functions.
+ *
+ * Ownership: the caller owns graphs; a finder borrows its graph (it must
+ * outlive the finder) and owns its device buffers.  A finder is used by one
+ * host thread at a time; distinct finders may run on distinct threads.
+ */
+#ifndef GMBE_H_
+#define GMBE_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GMBE_OK 0
+#define GMBE_ERR_ARG (-1)         /* invalid argument                       */
+#define GMBE_ERR_IO (-2)          /* file cannot be opened / written        */
+#define GMBE_ERR_FORMAT (-3)      /* malformed .adj input                   */
+#define GMBE_ERR_CAPACITY (-4)    /* a size exceeds a documented limit      */
+#define GMBE_ERR_HIP (-5)         /* HIP runtime error or no GPU            */
+#define GMBE_ERR_NOMEM (-6)       /* host or device allocation failed       */
+#define GMBE_ERR_REJECTED (-7)    /* serving admission rejected a request   */
+#define GMBE_ERR_UNSUPPORTED (-8) /* option not supported on this build     */
+#define GMBE_ERR_STATE (-9)       /* call out of order (e.g. list before run) */
+
+typedef struct gmbe_graph gmbe_graph;
+typedef struct gmbe_finder gmbe_finder;
+typedef struct gmbe_batch gmbe_batch;
+typedef struct gmbe_server gmbe_server;
+
+/* == Option (src/BiGraph.h:13-22); defaults uvtrans=0, order=1, fast_mode=0. */
+typedef struct {
+  int32_t uvtrans;   /* 0 keep sides; 1 swap if |U|<|V|; 2 swap if |U|>|V|  */
+  int32_t order;     /* 0 none; 1 degree asc; 2 degree desc; 3 unilateral   */
+  int32_t fast_mode; /* 1 = skip the 2-hop statistics pass                  */
+} gmbe_option;
+
+/* Result of one enumeration.  Counts are 64-bit (the reference's int32
+ * maximal_nodes_ wraps at 2^31, src/BicliqueFinder.h:28).  The digest is the
+ * order-independent hash defined in gmbe_hash_biclique(). */
+typedef struct {
+  uint64_t maximal;     /* maximal bicliques                                 */
+  uint64_t non_maximal; /* diagnostic: rejected task roots + rejected pushes */
+  uint64_t hash_sum;    /* sum of per-biclique hashes mod 2^64               */
+  uint64_t hash_xor;    /* xor of per-biclique hashes                        */
+  uint64_t nodes;       /* enumeration-tree nodes evaluated (roots + pushes) */
+  uint64_t tasks;       /* tasks run (top-level vertices + split children)   */
+  uint64_t splits;      /* tasks that were split into children               */
+  uint64_t overflow;    /* tasks that exceeded a capacity (result invalid)   */
+  uint64_t alg_bytes;   /* algorithmic bytes: 4 (deg(x) + |L_parent| + |C|)
+                           per push, 4 (deg(S) + sum deg(L)) per task root   */
+  double exe_time_s;    /* Execute start -> count on host (exe_time_)        */
+  double kernel_time_s; /* enumeration kernel time from HIP events           */
+} gmbe_result;
+
+/* ---- graphs ------------------------------------------------------------ */
+
+/* Default Option values (src/BiGraph.h:17-21). */
+void gmbe_option_default(gmbe_option* opt);
+
+/* Replaces CSRBiGraph::Read(const char*, Option) (src/BiGraph.cu:13-278).
+ * Line i of the file is V vertex i; its tokens are U ids; |U| = max id + 1.
+ * Lines are sorted and de-duplicated on load. */
+int gmbe_graph_read_adj(const char* path, const gmbe_option* opt,
+                        gmbe_graph** out);
+
+/* Same transform pipeline as gmbe_graph_read_adj, from an in-memory CSR in
+ * file orientation: row v (0 <= v < nV) lists U ids in [0, nU).  Rows need
+ * not be sorted.  Replaces the CSR build at src/BiGraph.cu:251-277. */
+int gmbe_graph_from_csr(uint32_t nU, uint32_t nV, const int64_t* v_off,
+                        const uint32_t* v_adj, const gmbe_option* opt,
+                        gmbe_graph** out);
+
+/* Apply an Option to an already loaded graph (the result keeps original ids
+ * of the source graph).  Used to derive the working graph of a generated one. */
+int gmbe_graph_derive(const gmbe_graph* src, const gmbe_option* opt,
+                      gmbe_graph** out);
+
+/* Statistics in internal orientation: out = {U_size, V_size, edge_size,
+ * U_degree, V_degree, U_2degree, V_2degree, swapped}.  The 2-hop degrees are
+ * -1 when the graph was loaded with fast_mode (src/BiGraph.h:30). */
+int gmbe_graph_stats(const gmbe_graph* g, int64_t out[8]);
+
+/* Borrowed pointers to the host CSR (internal orientation; lists ascending):
+ * V->U (row_offset/column_indices) and U->V (rev_row_offset/rev_column_indices),
+ * src/BiGraph.h:32-36; plus internal->original id maps of each side.  Any
+ * pointer argument may be NULL. */
+int gmbe_graph_csr(const gmbe_graph* g, const int64_t** v_off,
+                   const uint32_t** v_adj, const int64_t** u_off,
+                   const uint32_t** u_adj, const uint32_t** u_orig,
+                   const uint32_t** v_orig);
+
+/* Write the graph in .adj format (one line per internal V vertex). */
+int gmbe_graph_write_adj(const gmbe_graph* g, const char* path);
+
+/* Replaces CSRBiGraph::Reset() of a host graph (src/BiGraph.cu:433-454). */
+void gmbe_graph_destroy(gmbe_graph* g);
+
+/* ---- seeded synthetic workloads -------------------------------------- */
+
+/* Generators of the benchmark graphs.  Output is a graph in file orientation
+ * (no side swap, no reordering; original id == internal id).
+ *   kind 1: uniform bipartite, nU x nV, m distinct edges
+ *   kind 2: Chung-Lu power law, exponents gamma_u/gamma_v, degree cap
+ *   kind 3: kind 2 plus `blocks` planted a x b near-bicliques, a,b uniform in
+ *           [block_min, block_max], with round(drop*a*b) block edges removed
+ * Every vertex gets degree >= 1 before the remaining edges are drawn. */
+typedef struct {
+  int32_t kind;
+  uint32_t nU, nV;
+  uint64_t m;
+  double gamma_u, gamma_v;
+  uint32_t cap;
+  uint32_t blocks, block_min, block_max;
+  double drop;
+  uint64_t seed;
+} gmbe_gen_params;
+
+int gmbe_generate(const gmbe_gen_params* p, gmbe_graph** out);
+
+/* One graph of the batched workload: per-side sizes log-uniform in
+ * [lo, hi], mean degree uniform in [dlo, dhi], Chung-Lu exponent gamma. */
+int gmbe_generate_small(uint64_t seed, uint32_t lo, uint32_t hi, double dlo,
+                        double dhi, double gamma, gmbe_graph** out);
+
+/* 2-hop ego subgraph of original-U vertex u of a file-orientation graph:
+ * V' = N(u), all edges of V' in ascending V' id until max_edges is reached.
+ * The result is in file orientation with ids relabelled densely (ascending). */
+int gmbe_ego_subgraph(const gmbe_graph* g, uint32_t u, uint64_t max_edges,
+                      gmbe_graph** out);
+
+/* ---- hashing ------------------------------------------------------------ */
+
+/* Hash of one biclique (A = its vertices on the original U side, B = on the
+ * original V side, original ids, any order):
+ *   key(side, x) = mix64((x << 1) | side)            side 0 = U, 1 = V
+ *   SA = sum key(0, a), SB = sum key(1, b)   (mod 2^64)
+ *   h  = mix64(SA ^ rotl64(SB, 31))
+ * mix64 is the splitmix64 output function (Steele, Lea, Flood 2014).  The
+ * digest of an enumeration is (count, sum h mod 2^64, xor h). */
+uint64_t gmbe_hash_biclique(const uint32_t* A, size_t nA, const uint32_t* B,
+                            size_t nB);
+
+/* ---- the hot path: plain finder --------------------------------------- */
+
+/* Replaces `new IterFinderGpu6(graph, bound_height, bound_size)`
+ * (src/IterFinderGpu.h:77-90, src/IterFinderGpu.cu:1056-1085) and, for
+ * sel 0/1, IterFinderGpu / IterFinderGpu2 (src/main.cu:136-141): the sel value
+ * picks the scheduler (0: one wave per top-level vertex, no splitting;
+ * 1: same with larger per-wave prefixes; 2: load-aware splitting, the hot
+ * path).  sel 3 (multi-GPU, IterFinderGpu7) returns GMBE_ERR_UNSUPPORTED
+ * unless ngpus == 1.  Copies the graph to the device (CopyToGpu,
+ * src/BiGraph.cu:413-431). */
+int gmbe_finder_create(const gmbe_graph* g, int sel, int bound_height,
+                       int bound_size, int ngpus, gmbe_finder** out);
+
+/* Tuning knobs, set before execute (the reference's compile-time macros,
+ * src/CMakeLists.txt:13-39, src/GpuUtil.cuh:9-24, src/IterFinderGpu.cu:912-917):
+ *   "waves_per_block", "blocks", "queue_cap", "list_cap" (u32 words of list
+ *   output; 0 = off), "prune" (0/1, PRUNE_EN), "max_prefix" (2..8). */
+int gmbe_finder_set(gmbe_finder* f, const char* key, int64_t value);
+
+/* Replaces IterFinderGpu6::Execute (src/IterFinderGpu.cu:1095-1119):
+ * synchronous; exe_time_s is measured like exe_time_. */
+int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res);
+
+/* List mode (no reference counterpart: the reference never emits pairs).
+ * After an execute with list_cap > 0, copies records
+ *   [nA, nB, A[0..nA), B[0..nB)]   (original ids, A on the original U side)
+ * into buf.  *used receives the number of words required; returns
+ * GMBE_ERR_CAPACITY if cap is too small or the device buffer overflowed. */
+int gmbe_finder_list(gmbe_finder* f, uint32_t* buf, size_t cap, size_t* used);
+
+/* Replaces the finder destructor chain (src/IterFinderGpu.cu:1087-1093). */
+void gmbe_finder_destroy(gmbe_finder* f);
+
+/* ---- batched form ------------------------------------------------------ */
+
+/* Many independent graphs enumerated by one persistent launch (captured in a
+ * HIP graph and replayed by gmbe_batch_execute).  No reference counterpart
+ * (the reference runs one graph per process). */
+int gmbe_batch_create(const gmbe_graph* const* graphs, int n, int bound_height,
+                      int bound_size, gmbe_batch** out);
+int gmbe_batch_set(gmbe_batch* b, const char* key, int64_t value);
+/* per_graph may be NULL; otherwise it receives n results (times are zero). */
+int gmbe_batch_execute(gmbe_batch* b, gmbe_result* per_graph,
+                       gmbe_result* total);
+void gmbe_batch_destroy(gmbe_batch* b);
+
+/* ---- serving loop ------------------------------------------------------ */
+
+typedef struct {
+  uint64_t ticket;      /* id returned by submit                      */
+  int32_t status;       /* 0 ok, negative error                        */
+  uint64_t maximal, hash_sum, hash_xor;
+  double t_submit_s, t_done_s; /* monotonic clock of submit / completion */
+} gmbe_serve_done;
+
+/* A server keeps `resident` (file orientation) on the device and runs
+ * requests with continuous batching: requests are admitted while slots and
+ * the device memory budget (bytes) allow, and their work overlaps on the GPU.
+ * No reference counterpart. */
+int gmbe_serve_create(const gmbe_graph* resident, int slots,
+                      uint64_t mem_budget, int bound_height, int bound_size,
+                      gmbe_server** out);
+/* Submit a stand-alone graph (copied; file orientation). */
+int gmbe_serve_submit_graph(gmbe_server* s, const gmbe_graph* g,
+                            uint64_t* ticket);
+/* Submit the 2-hop ego subgraph of original-U vertex u of the resident graph. */
+int gmbe_serve_submit_ego(gmbe_server* s, uint32_t u, uint64_t max_edges,
+                          uint64_t* ticket);
+/* Collect finished requests (non-blocking unless wait != 0). */
+int gmbe_serve_poll(gmbe_server* s, gmbe_serve_done* out, int max, int wait,
+                    int* n_out);
+void gmbe_serve_destroy(gmbe_server* s);
+
+/* ---- misc ---------------------------------------------------------------- */
+
+/* Thread-local message of the last failing call on this thread. */
+const char* gmbe_last_error(void);
+
+/* Device name / CU count of device 0 into buf; GMBE_ERR_HIP without a GPU. */
+int gmbe_device_info(char* buf, size_t cap, int* cu_count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GMBE_H_ */

@@ -1,0 +1,7 @@
+"""CPU oracle for the maximal-biclique enumeration — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this
+package, as the checker; the engine (gmbe_amd) never imports it.
+"""
+from .binding import (OracleResult, available, brute, gmbe, mbea, hash_pair,  # noqa: F401
+                      LIB_PATH)

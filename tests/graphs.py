@@ -1,0 +1,97 @@
+"""Graphs used by the tests: closed-form families and seeded random graphs.
+
+Each family comes with its known maximal-biclique count (and, for the crown
+graph, the full list).  Graphs are given in file orientation: rows = V.
+"""
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+
+from gmbe_amd import BiGraph
+
+
+def complete(a, b):
+    """K_{a,b}: exactly one maximal biclique."""
+    return a, [list(range(a)) for _ in range(b)], 1
+
+
+def matching(n):
+    """n disjoint edges: n bicliques."""
+    return n, [[i] for i in range(n)], n
+
+
+def crown(n):
+    """K_{n,n} minus a perfect matching: 2^n - 2 bicliques (n >= 2)."""
+    return n, [[u for u in range(n) if u != v] for v in range(n)], (2 ** n - 2 if n >= 2 else 0)
+
+
+def crown_list(n):
+    """Closed form: for every non-empty proper subset S of [n],
+    (L, R) = ({u_i : i in S}, {v_j : j not in S})."""
+    out = set()
+    for r in range(1, n):
+        for S in itertools.combinations(range(n), r):
+            rest = tuple(j for j in range(n) if j not in S)
+            out.add((tuple(S), rest))
+    return out
+
+
+def chain(k):
+    """Nested neighbourhoods N(v_0) ⊂ ... ⊂ N(v_{k-1}): k bicliques."""
+    return k, [list(range(i + 1)) for i in range(k)], k
+
+
+def star_v(n):
+    """One V vertex adjacent to n U vertices: 1 biclique."""
+    return n, [list(range(n))], 1
+
+
+def star_u(n):
+    """One U vertex adjacent to n V vertices: 1 biclique."""
+    return 1, [[0] for _ in range(n)], 1
+
+
+def disjoint(*parts):
+    """Disjoint union of (nU, rows, count) families: counts add."""
+    nU, rows, cnt = 0, [], 0
+    for pu, prow, pc in parts:
+        rows.extend([[u + nU for u in r] for r in prow])
+        nU += pu
+        cnt += pc
+    return nU, rows, cnt
+
+
+def random_rows(nU, nV, p, seed):
+    rng = np.random.default_rng(seed)
+    rows = []
+    for _ in range(nV):
+        r = sorted(int(u) for u in np.nonzero(rng.random(nU) < p)[0])
+        if not r:
+            r = [int(rng.integers(nU))]
+        rows.append(r)
+    return rows
+
+
+KATS = {
+    "K_3x4": complete(3, 4),
+    "K_1x1": complete(1, 1),
+    "matching_7": matching(7),
+    "crown_2": crown(2),
+    "crown_5": crown(5),
+    "crown_9": crown(9),
+    "chain_6": chain(6),
+    "star_v_10": star_v(10),
+    "star_u_10": star_u(10),
+    "union": disjoint(complete(3, 2), crown(4), matching(3), chain(4)),
+}
+
+
+def kat_graph(name, **opt):
+    nU, rows, cnt = KATS[name]
+    return BiGraph.from_rows(nU, rows, **opt), cnt
+
+
+def random_graph(nU, nV, p, seed, **opt):
+    return BiGraph.from_rows(nU, random_rows(nU, nV, p, seed), **opt)

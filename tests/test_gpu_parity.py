@@ -1,0 +1,163 @@
+"""Parity of the HIP enumerator (through the C ABI) with the CPU oracle.
+
+Bit-exact: the count and the (sum, xor) hash digest of every maximal biclique
+must equal the oracle's; on small graphs the full list of (L, R) pairs is
+compared.  Full-size graphs are checked on root samples (the digest over a
+sample of top-level vertices is exact and additive) and through invariances.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import graphs
+import oracle
+from gmbe_amd import BiGraph, GmbeError, IterFinderGpu6, digest_of, enumerate_graph, workloads
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def gpu_digest(g, **kw):
+    r = enumerate_graph(g, **kw)
+    assert r.overflow == 0
+    return r
+
+
+def gpu_list(g, cap=1 << 22, **kw):
+    f = IterFinderGpu6(g, list_cap=cap, **kw)
+    r = f.execute()
+    s = f.bicliques()
+    f.close()
+    return r, s
+
+
+@pytest.mark.parametrize("name", sorted(graphs.KATS))
+def test_closed_forms(gpu, name):
+    g, cnt = graphs.kat_graph(name, uvtrans=1, order=1, fast_mode=1)
+    r = gpu_digest(g)
+    assert r.maximal == cnt
+    assert r.digest() == oracle.gmbe(g.oracle_graph()).digest()
+
+
+@pytest.mark.parametrize("n", [2, 5, 9, 12])
+def test_crown_list(gpu, n):
+    g = BiGraph.from_rows(*graphs.crown(n)[:2], uvtrans=1, order=1, fast_mode=1)
+    r, s = gpu_list(g)
+    assert s == graphs.crown_list(n)
+    assert r.digest() == digest_of(s)
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_random_small_lists(gpu, seed):
+    rng = np.random.default_rng(seed)
+    nU, nV = int(rng.integers(4, 50)), int(rng.integers(3, 20))
+    g = graphs.random_graph(nU, nV, float(rng.uniform(0.1, 0.6)), 2000 + seed)
+    g = g.derive(uvtrans=1, order=1, fast_mode=1)
+    res, lst = oracle.brute(g.oracle_graph(), want_list=True) if nV <= 24 and nU <= 64 \
+        else oracle.gmbe(g.oracle_graph(), want_list=True)
+    from gmbe_amd import records_to_set
+    expect = records_to_set(lst)
+    for sel, bh, bs in [(2, 20, 1500), (0, 20, 1500), (2, 0, 0)]:
+        r, s = gpu_list(g, sel=sel, bound_height=bh, bound_size=bs)
+        assert s == expect, (sel, bh, bs)
+        assert r.digest() == res.digest()
+
+
+def test_config1_full_list(gpu):
+    g = workloads.run_graph(1)
+    res, lst = oracle.gmbe(g.oracle_graph(), want_list=True)
+    from gmbe_amd import records_to_set
+    expect = records_to_set(lst)
+    assert len(expect) == 3988  # reference CPU count (tests/golden/ref_counts.json)
+    r, s = gpu_list(g)
+    assert s == expect
+    assert r.digest() == res.digest() == digest_of(expect)
+
+
+@pytest.mark.parametrize("sel,bh,bs,mp", [(2, 20, 1500, 4), (0, 20, 1500, 4), (2, 0, 0, 4),
+                                          (2, 0, 0, 8), (2, 5, 50, 3)])
+def test_config1_schedulers(gpu, sel, bh, bs, mp):
+    g = workloads.run_graph(1)
+    base = oracle.gmbe(g.oracle_graph()).digest()
+    r = gpu_digest(g, sel=sel, bound_height=bh, bound_size=bs, max_prefix=mp)
+    assert r.digest() == base
+
+
+@pytest.mark.parametrize("lds_kb", [0, 4, 8, 16, 24])
+def test_lds_tiers(gpu, lds_kb):
+    g = workloads.run_graph(2, 0.01)
+    base = oracle.gmbe(g.oracle_graph(), threads=8).digest()
+    assert gpu_digest(g, lds_kb=lds_kb).digest() == base
+
+
+REF = json.load(open(os.path.join(HERE, "golden", "ref_counts.json")))["graphs"]
+
+
+@pytest.mark.parametrize("name", ["config2_s0.02", "config3_s0.002"])
+def test_scaled_configs_vs_reference_counts(gpu, name):
+    rec = REF[name]["recipe"]
+    g = workloads.run_graph(rec["config"], rec["scale"])
+    r = gpu_digest(g)
+    assert r.maximal == REF[name]["count"]
+    assert r.digest() == oracle.gmbe(g.oracle_graph(), threads=8).digest()
+
+
+def test_wide_bitsets(gpu):
+    """|L| > 256 without splitting goes through the wide-bitset pass."""
+    g = graphs.random_graph(700, 24, 0.45, 31).derive(uvtrans=1, order=1, fast_mode=1)
+    assert g.stats()["V_size"] == 24
+    base = oracle.gmbe(g.oracle_graph(), threads=8).digest()
+    for sel in (0, 2):
+        assert gpu_digest(g, sel=sel).digest() == base
+
+
+def test_very_wide_l_must_split(gpu):
+    """|L| > 1024 is split into children whatever the bounds."""
+    g = graphs.random_graph(1500, 12, 0.7, 32).derive(uvtrans=1, order=1, fast_mode=1)
+    base = oracle.gmbe(g.oracle_graph(), threads=8).digest()
+    assert gpu_digest(g, sel=2, bound_height=1 << 20, bound_size=1 << 30).digest() == base
+
+
+def test_edge_cases(gpu):
+    empty = BiGraph.from_rows(3, [[], [], []], uvtrans=1, order=1, fast_mode=1)
+    assert gpu_digest(empty).maximal == 0
+    single = BiGraph.from_rows(1, [[0]], uvtrans=1, order=1, fast_mode=1)
+    assert gpu_digest(single).maximal == 1
+    twins = BiGraph.from_rows(4, [[0, 1], [0, 1], [1, 2, 3], [0, 1]], uvtrans=0, order=1,
+                              fast_mode=1)
+    assert gpu_digest(twins).digest() == oracle.brute(twins.oracle_graph()).digest()
+    isolated = BiGraph.from_rows(3, [[0, 1], [], [1, 2]], uvtrans=0, order=0, fast_mode=1)
+    assert gpu_digest(isolated).digest() == oracle.brute(isolated.oracle_graph()).digest()
+
+
+def test_capacity_error_is_reported(gpu):
+    """A per-wave arena too small for a task is an error, never a silent miss."""
+    g = workloads.run_graph(2, 0.02)
+    f = IterFinderGpu6(g, arena_kb=64, lds_kb=0)
+    try:
+        r = f.execute()
+    except GmbeError as e:
+        assert e.code == -4
+    else:
+        assert r.overflow == 0
+        assert r.maximal == REF["config2_s0.02"]["count"]
+    f.close()
+
+
+def test_repeated_execute_is_identical(gpu):
+    g = workloads.run_graph(2, 0.01)
+    f = IterFinderGpu6(g)
+    a = f.execute().digest()
+    b = f.execute().digest()
+    f.close()
+    assert a == b
+
+
+@pytest.mark.parametrize("stride,phase", [(2000, 1999), (2000, 1500), (5000, 4000)])
+def test_full_size_config2_root_sample(gpu, stride, phase):
+    g = workloads.run_graph(2)
+    res = oracle.gmbe(g.oracle_graph(), threads=8, stride=stride, phase=phase)
+    r = gpu_digest(g, root_stride=stride, root_phase=phase)
+    assert r.digest() == res.digest()

@@ -1,0 +1,154 @@
+"""Pins the CPU oracle (test infrastructure) before it is trusted.
+
+* counts of the reference's own CPU code (baselines/MBE.zip built into
+  oracle/_ref), frozen in tests/golden/ref_counts.json;
+* closed forms (complete, matching, crown = 2^n - 2 with its full list, chain,
+  stars, disjoint unions);
+* the definition itself: brute force over every subset of V on small graphs;
+* three-way agreement of the GMBE-order restatement, the MBEA restatement and
+  brute force, with full lists;
+* invariance of the digest under side swap, vertex order, task splitting and
+  PRUNE_EN (all change the traversal, none may change the result).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import graphs
+import oracle
+from gmbe_amd import BiGraph, digest_of, hash_biclique, workloads
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+M64 = (1 << 64) - 1
+
+
+def _mix64(x):
+    z = (x + 0x9E3779B97F4A7C15) & M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return z ^ (z >> 31)
+
+
+def _hash_py(A, B):
+    sa = sum(_mix64((a << 1) | 0) for a in A) & M64
+    sb = sum(_mix64((b << 1) | 1) for b in B) & M64
+    rot = ((sb << 31) | (sb >> 33)) & M64
+    return _mix64(sa ^ rot)
+
+
+def test_hash_known_answers():
+    kat = json.load(open(os.path.join(HERE, "golden", "hash_kat.json")))
+    for case in kat:
+        A, B, h = case["A"], case["B"], case["h"]
+        assert oracle.hash_pair(A, B) == h
+        assert hash_biclique(A, B) == h
+        assert _hash_py(A, B) == h
+    # order independence within a side, side sensitivity
+    assert hash_biclique([3, 1, 2], [9, 8]) == hash_biclique([1, 2, 3], [8, 9])
+    assert hash_biclique([1], [2]) != hash_biclique([2], [1])
+
+
+def build_recipe(rec):
+    k = rec["kind"]
+    if k == "kat":
+        g, _ = graphs.kat_graph(rec["name"])
+        return g
+    if k == "random":
+        return graphs.random_graph(rec["nU"], rec["nV"], rec["p"], rec["seed"])
+    if k == "config":
+        return workloads.raw_graph(rec["config"], rec["scale"])
+    raise ValueError(k)
+
+
+REF = json.load(open(os.path.join(HERE, "golden", "ref_counts.json")))["graphs"]
+
+
+@pytest.mark.parametrize("name", sorted(REF))
+def test_oracle_matches_reference_cpu_counts(name):
+    """Counts of the reference's own MbeaAdvFinder / ImbeaAdvFinder / PmbeAdvFinder."""
+    ent = REF[name]
+    g = build_recipe(ent["recipe"]).derive(uvtrans=1, order=1, fast_mode=1)
+    og = g.oracle_graph()
+    assert oracle.gmbe(og, threads=8).maximal == ent["count"]
+    if ent["count"] < 5_000_000:
+        assert oracle.mbea(og, threads=8).maximal == ent["count"]
+
+
+@pytest.mark.parametrize("name", sorted(graphs.KATS))
+def test_closed_forms(name):
+    g, cnt = graphs.kat_graph(name)
+    og = g.oracle_graph()
+    for run in (oracle.gmbe, oracle.mbea):
+        assert run(og).maximal == cnt
+    if g.stats()["V_size"] <= 24 and g.stats()["U_size"] <= 64:
+        assert oracle.brute(og).maximal == cnt
+
+
+@pytest.mark.parametrize("n", [2, 3, 6, 10])
+def test_crown_graph_full_list(n):
+    g = BiGraph.from_rows(*graphs.crown(n)[:2])
+    expect = graphs.crown_list(n)
+    for run in (oracle.gmbe, oracle.mbea):
+        res, lst = run(g.oracle_graph(), want_list=True)
+        from gmbe_amd import records_to_set
+        got = records_to_set(lst)
+        assert got == expect
+        assert res.digest() == digest_of(expect)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_three_way_agreement_small(seed):
+    rng = np.random.default_rng(seed)
+    nU, nV = int(rng.integers(4, 40)), int(rng.integers(3, 17))
+    p = float(rng.uniform(0.1, 0.6))
+    g = graphs.random_graph(nU, nV, p, 1000 + seed)
+    og = g.oracle_graph()
+    from gmbe_amd import records_to_set
+    rb, lb = oracle.brute(og, want_list=True)
+    rg, lg = oracle.gmbe(og, want_list=True)
+    rm, lm = oracle.mbea(og, want_list=True)
+    sb = records_to_set(lb)
+    assert records_to_set(lg) == sb
+    assert records_to_set(lm) == sb
+    assert rb.digest() == rg.digest() == rm.digest() == digest_of(sb)
+
+
+@pytest.mark.parametrize("opt", [(0, 0), (1, 1), (2, 2), (1, 0), (0, 3)])
+def test_digest_invariant_to_orientation_and_order(opt):
+    raw = graphs.random_graph(60, 45, 0.12, 77)
+    base = oracle.gmbe(raw.derive(uvtrans=0, order=0, fast_mode=0).oracle_graph()).digest()
+    g = raw.derive(uvtrans=opt[0], order=opt[1], fast_mode=0 if opt[1] == 3 else 1)
+    assert oracle.gmbe(g.oracle_graph()).digest() == base
+    assert oracle.mbea(g.oracle_graph()).digest() == base
+
+
+@pytest.mark.parametrize("bh,bs,mp", [(0, 0, 4), (0, 0, 8), (3, 10, 4), (20, 1500, 4)])
+def test_split_rule_keeps_the_result(bh, bs, mp):
+    g = workloads.run_graph(1)
+    og = g.oracle_graph()
+    base = oracle.gmbe(og).digest()
+    split = oracle.gmbe(og, bound_height=bh, bound_size=bs, max_prefix=mp)
+    assert split.digest() == base
+
+
+def test_prune_keeps_the_result_and_saves_pushes():
+    og = workloads.run_graph(1).oracle_graph()
+    a = oracle.gmbe(og, prune=1)
+    b = oracle.gmbe(og, prune=0)
+    assert a.digest() == b.digest()
+    assert a.nodes <= b.nodes and a.non_maximal <= b.non_maximal
+
+
+def test_root_sample_partition():
+    """Digests over disjoint root samples add up to the full digest."""
+    og = workloads.run_graph(2, 0.01).oracle_graph()
+    full = oracle.gmbe(og, threads=8)
+    parts = [oracle.gmbe(og, threads=8, stride=3, phase=p) for p in range(3)]
+    assert sum(p.maximal for p in parts) == full.maximal
+    assert sum(p.hash_sum for p in parts) % (1 << 64) == full.hash_sum
+    x = 0
+    for p in parts:
+        x ^= p.hash_xor
+    assert x == full.hash_xor
