@@ -27,6 +27,7 @@ struct DeviceGraph {
   mbe::DevGraph view() const;
 };
 
-mbe::Scratch scratch_layout(uint32_t nV, uint32_t dmaxV, uint64_t arena_bytes);
+mbe::Scratch scratch_layout(uint32_t nV, uint32_t dmaxV, uint64_t arena_bytes,
+                            uint64_t ctx_bytes);
 
 }  // namespace gmbe

@@ -33,19 +33,33 @@ __global__ __launch_bounds__(WAVES * 64) void mbe_persistent(Params P) {
   int k = 0;
   uint32_t word = 0;
   for (;;) {
-    int kind = next_work(P, word, t, k);
-    if (kind == 0) break;
+    uint64_t tw = P.profile ? cycles() : 0;
+    int kind = next_work(P, word, t, k, (uint32_t)(gw & (kShards - 1)));
+    if (kind == 0) {
+      // the final wait (nothing left for this wave) goes to the split slot's
+      // neighbour so that mid-run waiting and the tail can be told apart
+      if (P.profile) c.tail += cycles() - tw;
+      break;
+    }
+    phase(c, P.profile, 0, tw);
     if (kind == 2) {
-      run_task<WMAX>(P, m, t, k, c);
+      Prefix pre{nullptr, 0, 0};
+      if (k > 1) pre = prefix_common(P.g, m, t, k - 1);
+      run_task<WMAX>(P, m, t, k, pre, c, gw);
+    } else if (read_lane(word, 0) & kItemKind) {
+      // items of a shared task context
+      run_items<WMAX>(P, m, word, c, gw);
     } else {
-      // an entry: children prefix + [c_i] of one split task
+      // an entry: children prefix + [c_i] of one split task; the prefix's
+      // common neighbourhood is computed once for all of them
       const uint32_t hdr = read_lane(word, 0);
       const int kp = (int)(hdr & 0xffu);
       const uint32_t cnt = hdr >> 8;
       for (int i = 0; i < kMaxPrefix; ++i) t.v[i] = i < kp ? read_lane(word, 1 + i) : kNone;
+      const Prefix pre = prefix_common(P.g, m, t, kp);
       for (uint32_t i = 0; i < cnt; ++i) {
         t.v[kp] = read_lane(word, 8 + i);
-        run_task<WMAX>(P, m, t, kp + 1, c);
+        run_task<WMAX>(P, m, t, kp + 1, pre, c, gw);
       }
     }
     work_done(P);
@@ -70,14 +84,42 @@ __global__ __launch_bounds__(WAVES * 64) void mbe_persistent(Params P) {
                            __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_fetch_add(a + A_BALG, (unsigned long long)c.balg, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
+    if (P.profile) {
+      for (int i = 0; i < 7; ++i)
+        __hip_atomic_fetch_add(a + A_CYC_WAIT + i, (unsigned long long)c.cyc[i], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(a + A_EDGES, (unsigned long long)c.edges, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(a + A_ENTRIES, (unsigned long long)c.entries, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(a + A_ENT_W1, (unsigned long long)c.ent_w1, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(a + A_NODES_W1, (unsigned long long)c.nodes_w1, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(a + A_CYC_TAIL, (unsigned long long)c.tail, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+      const uint64_t extra[6] = {c.c_pass, c.c_merge, c.c_compress, c.c_publish, c.publishes,
+                                 c.merges};
+      for (int i = 0; i < 6; ++i)
+        __hip_atomic_fetch_add(a + A_CYC_PASS + i, (unsigned long long)extra[i],
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __hip_atomic_fetch_add(a + A_OVF_ARENA, (unsigned long long)c.ovf_arena, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(a + A_OVF_WIDE, (unsigned long long)c.ovf_wide, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(a + A_OVF_DEFER, (unsigned long long)c.ovf_defer, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
-// Reset the queue's slot sequence numbers (slot i starts free for position i).
+// Reset the queue's slot sequence numbers (slot i of every shard starts free
+// for position i).
 __global__ void init_queue(unsigned long long* seq, unsigned long long cap) {
-  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < cap;
-       i += (unsigned long long)gridDim.x * blockDim.x)
-    seq[i] = i;
+  const unsigned long long total = cap * kShards;
+  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+       i < total; i += (unsigned long long)gridDim.x * blockDim.x)
+    seq[i] = i & (cap - 1);
 }
 
 }  // namespace mbe
@@ -95,9 +137,11 @@ namespace gmbe {
 typedef void (*KernelFn)(mbe::Params);
 
 // control counters live kCtlStride u64 (256 bytes) apart so that polling one
-// never contends with atomics on another
-constexpr int kCtlStride = 32;
-enum CtlSlot { C_ROOT_NEXT = 0, C_WORK_LEFT, C_QHEAD, C_QTAIL, C_LIST_CURSOR, C_DEFERRED, C_NSLOTS };
+// never contends with atomics on another; the shards' head/tail pairs follow
+using mbe::kCtlStride;
+using mbe::kShards;
+enum CtlSlot { C_ROOT_NEXT = 0, C_WORK_LEFT, C_QUEUED, C_LIST_CURSOR, C_DEFERRED, C_QCTL = 8,
+               C_NSLOTS = 8 + 2 * kShards };
 
 struct KernelChoice {
   KernelFn fn;
@@ -111,6 +155,7 @@ static KernelChoice pick_kernel(int lds_kb, bool wide = false) {
   switch (lds_kb) {
     case 0: return {mbe_persistent<4, 0, kMainW>, 4, 0};
     case 4: return {mbe_persistent<4, 4096, kMainW>, 4, 4096};
+    case 12: return {mbe_persistent<4, 12288, kMainW>, 4, 12288};
     case 16: return {mbe_persistent<4, 16384, kMainW>, 4, 16384};
     case 24: return {mbe_persistent<4, 24576, kMainW>, 4, 24576};
     default: return {mbe_persistent<4, 8192, kMainW>, 4, 8192};
@@ -192,7 +237,8 @@ static uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
 // Per-wave slab layout for a graph with nV enumeration-side vertices and
 // maximum enumeration-side degree dmaxV.
-mbe::Scratch scratch_layout(uint32_t nV, uint32_t dmaxV, uint64_t arena_bytes) {
+mbe::Scratch scratch_layout(uint32_t nV, uint32_t dmaxV, uint64_t arena_bytes,
+                            uint64_t ctx_bytes) {
   mbe::Scratch s{};
   uint64_t o = 0;
   auto take = [&](uint64_t bytes) {
@@ -218,6 +264,8 @@ mbe::Scratch scratch_layout(uint32_t nV, uint32_t dmaxV, uint64_t arena_bytes) {
   s.o_fr = take((d + 2) * sizeof(mbe::Frame));
   s.o_arena = take(arena_bytes);
   s.arena_bytes = arena_bytes;
+  s.o_ctx = take(ctx_bytes);
+  s.ctx_bytes = ctx_bytes;
   s.slab = o;
   return s;
 }
@@ -231,7 +279,10 @@ struct gmbe_finder {
   int sel = 2, bound_height = 20, bound_size = 1500;
   // knobs
   int64_t waves_per_block = 4, blocks = 0, prune = 1, max_prefix = 4, lds_kb = 8;
-  int64_t queue_log2 = 22, list_cap = 0, arena_kb = 4096, root_stride = 1, root_phase = 0;
+  int64_t queue_log2 = 22, list_cap = 0, arena_kb = 16384, root_stride = 1, root_phase = 0;
+  int64_t profile = 0, group = 1, share = 1, ctx_kb = 4096, share_work = 4096, share_queue = -1;
+  int64_t merge_min = 256, merge_branch = 16;
+  unsigned long long prof[mbe::A_COUNT] = {};
   gmbe::Engine eng;
   gmbe::DeviceGraph dg;
   // device buffers
@@ -243,7 +294,7 @@ struct gmbe_finder {
   unsigned long long* qseq = nullptr;
   uint64_t qcap = 0;
   // control counters, each on its own 256-byte line (kCtlStride u64 apart):
-  // root_next, work_left, qhead, qtail, list_cursor, n_deferred
+  // root_next, work_left, queued, list_cursor, n_deferred, then per-shard head/tail
   unsigned long long* ctl = nullptr;
   unsigned long long* acc = nullptr;
   uint32_t* list = nullptr;
@@ -267,8 +318,8 @@ static int finder_alloc(gmbe_finder* f) {
   if (per_sm < 1) per_sm = 1;
   int blocks = f->blocks > 0 ? (int)f->blocks : per_sm * f->eng.cu_count;
   int nwaves = blocks * kc.waves;
-  mbe::Scratch lay =
-      scratch_layout(f->dg.nV, f->dg.dmaxV, (uint64_t)f->arena_kb * 1024ull);
+  mbe::Scratch lay = scratch_layout(f->dg.nV, f->dg.dmaxV, (uint64_t)f->arena_kb * 1024ull,
+                                    (uint64_t)f->ctx_kb * 1024ull);
   uint64_t bytes = lay.slab * (uint64_t)nwaves;
   if (bytes != f->scratch_bytes || nwaves != f->nwaves) {
     if (f->scratch) (void)hipFree(f->scratch);
@@ -285,14 +336,15 @@ static int finder_alloc(gmbe_finder* f) {
   f->layout = lay;
   f->nwaves = nwaves;
   f->nblocks = blocks;
-  uint64_t qcap = 1ull << f->queue_log2;
+  uint64_t qcap = (1ull << f->queue_log2) / kShards;  // slots per shard
+  if (qcap < 16) qcap = 16;
   if (qcap != f->qcap) {
     if (f->q) (void)hipFree(f->q);
     if (f->qseq) (void)hipFree(f->qseq);
     f->q = nullptr;
     f->qseq = nullptr;
-    HIP_TRY(hipMalloc(&f->q, qcap * mbe::kEntryWords * sizeof(uint32_t)));
-    HIP_TRY(hipMalloc(&f->qseq, qcap * 8));
+    HIP_TRY(hipMalloc(&f->q, qcap * kShards * mbe::kEntryWords * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc(&f->qseq, qcap * kShards * 8));
     f->qcap = qcap;
   }
   if (!f->deferred) HIP_TRY(hipMalloc(&f->deferred, f->deferred_cap * sizeof(mbe::Task)));
@@ -334,7 +386,8 @@ int gmbe_finder_create(const gmbe_graph* g, int sel, int bound_height, int bound
     return set_error(GMBE_ERR_HIP, "stream/event creation failed");
   }
   rc = f->dg.upload(*g);
-  if (rc == GMBE_OK && hipMalloc(&f->ctl, 8 * kCtlStride * sizeof(unsigned long long)) != hipSuccess)
+  if (rc == GMBE_OK &&
+      hipMalloc(&f->ctl, C_NSLOTS * kCtlStride * sizeof(unsigned long long)) != hipSuccess)
     rc = set_error(GMBE_ERR_NOMEM, "control block");
   if (rc == GMBE_OK && hipMalloc(&f->acc, mbe::A_COUNT * sizeof(unsigned long long)) != hipSuccess)
     rc = set_error(GMBE_ERR_NOMEM, "result block");
@@ -361,14 +414,22 @@ int gmbe_finder_set(gmbe_finder* f, const char* key, int64_t value) {
   else if (k == "prune") { if ((rc = range(0, 1))) return rc; f->prune = value; }
   else if (k == "max_prefix") { if ((rc = range(2, mbe::kMaxPrefix))) return rc; f->max_prefix = value; }
   else if (k == "lds_kb") {
-    if (value != 0 && value != 4 && value != 8 && value != 16 && value != 24)
-      return set_error(GMBE_ERR_ARG, "lds_kb must be 0, 4, 8, 16 or 24");
+    if (value != 0 && value != 4 && value != 8 && value != 12 && value != 16 && value != 24)
+      return set_error(GMBE_ERR_ARG, "lds_kb must be 0, 4, 8, 12, 16 or 24");
     f->lds_kb = value;
   }
   else if (k == "queue_log2") { if ((rc = range(4, 30))) return rc; f->queue_log2 = value; }
   else if (k == "list_cap") { if ((rc = range(0, (int64_t)1 << 34))) return rc; f->list_cap = value; }
   else if (k == "arena_kb") { if ((rc = range(64, 1 << 20))) return rc; f->arena_kb = value; }
   else if (k == "root_stride") { if ((rc = range(1, 1 << 30))) return rc; f->root_stride = value; }
+  else if (k == "share_work") { if ((rc = range(1, 1ll << 40))) return rc; f->share_work = value; }
+  else if (k == "share_queue") { if ((rc = range(-1, 1ll << 40))) return rc; f->share_queue = value; }
+  else if (k == "merge_branch") { if ((rc = range(0, 1ll << 30))) return rc; f->merge_branch = value; }
+  else if (k == "merge_min") { if ((rc = range(2, 1ll << 30))) return rc; f->merge_min = value; }
+  else if (k == "share") { if ((rc = range(0, 1))) return rc; f->share = value; }
+  else if (k == "ctx_kb") { if ((rc = range(16, 1 << 20))) return rc; f->ctx_kb = value; }
+  else if (k == "group") { if ((rc = range(0, 1))) return rc; f->group = value; }
+  else if (k == "profile") { if ((rc = range(0, 1))) return rc; f->profile = value; }
   else if (k == "root_phase") { if ((rc = range(0, (1 << 30) - 1))) return rc; f->root_phase = value; }
   else if (k == "bound_height") { if ((rc = range(0, 1 << 30))) return rc; f->bound_height = (int)value; }
   else if (k == "bound_size") { if ((rc = range(0, (int64_t)1 << 30))) return rc; f->bound_size = (int)value; }
@@ -388,7 +449,7 @@ int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res) {
   double t0 = now_s();  // start_time_ (src/IterFinderGpu.cu:1096)
   uint64_t nV = f->dg.nV, stride = (uint64_t)f->root_stride, phase = (uint64_t)f->root_phase;
   uint64_t n_roots = nV > phase ? (nV - phase + stride - 1) / stride : 0;
-  std::vector<unsigned long long> ctl_h(8 * kCtlStride, 0ull);
+  std::vector<unsigned long long> ctl_h(C_NSLOTS * kCtlStride, 0ull);
   ctl_h[C_WORK_LEFT * kCtlStride] = n_roots;
   HIP_TRY(hipMemcpyAsync(f->ctl, ctl_h.data(), ctl_h.size() * 8, hipMemcpyHostToDevice,
                          f->stream));
@@ -400,8 +461,8 @@ int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res) {
   P.g = f->dg.view();
   P.s.root_next = f->ctl + C_ROOT_NEXT * kCtlStride;
   P.s.work_left = f->ctl + C_WORK_LEFT * kCtlStride;
-  P.s.qhead = f->ctl + C_QHEAD * kCtlStride;
-  P.s.qtail = f->ctl + C_QTAIL * kCtlStride;
+  P.s.qctl = f->ctl + C_QCTL * kCtlStride;
+  P.s.queued = f->ctl + C_QUEUED * kCtlStride;
   P.list_cursor = f->ctl + C_LIST_CURSOR * kCtlStride;
   P.s.n_deferred = f->ctl + C_DEFERRED * kCtlStride;
   P.s.deferred = f->deferred;
@@ -422,20 +483,27 @@ int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res) {
   P.prune = (int)f->prune;
   P.max_prefix = (int)f->max_prefix;
   P.split = f->sel == 2;
+  P.profile = (int)f->profile;
+  P.group = (int)f->group;
+  P.share = (int)f->share;
+  P.merge_min = (int)f->merge_min;
+  P.merge_branch = (int)f->merge_branch;
+  P.share_work = (int)std::min<int64_t>(f->share_work, INT32_MAX);
+  P.share_queue = f->share_queue >= 0 ? (int)f->share_queue : f->nwaves;
   KernelChoice kc = pick_kernel((int)f->lds_kb);
   HIP_TRY(hipEventRecord(f->ev0, f->stream));
   if (n_roots > 0) {
     hipLaunchKernelGGL(kc.fn, dim3(f->nblocks), dim3(kc.waves * 64), 0, f->stream, P);
     HIP_TRY(hipGetLastError());
   }
-  std::vector<unsigned long long> ctl_out(8 * kCtlStride, 0ull);
+  std::vector<unsigned long long> ctl_out(C_NSLOTS * kCtlStride, 0ull);
   HIP_TRY(hipMemcpyAsync(ctl_out.data(), f->ctl, ctl_out.size() * 8, hipMemcpyDeviceToHost,
                          f->stream));
   HIP_TRY(hipStreamSynchronize(f->stream));
   f->last_deferred = std::min<uint64_t>(ctl_out[C_DEFERRED * kCtlStride], f->deferred_cap);
   if (f->last_deferred > 0) {
     // wide-bitset pass over the deferred tasks (same queue, fresh counters)
-    std::vector<unsigned long long> ctl2(8 * kCtlStride, 0ull);
+    std::vector<unsigned long long> ctl2(C_NSLOTS * kCtlStride, 0ull);
     ctl2[C_WORK_LEFT * kCtlStride] = f->last_deferred;
     ctl2[C_LIST_CURSOR * kCtlStride] = ctl_out[C_LIST_CURSOR * kCtlStride];
     HIP_TRY(hipMemcpyAsync(f->ctl, ctl2.data(), ctl2.size() * 8, hipMemcpyHostToDevice,
@@ -473,11 +541,14 @@ int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res) {
   f->ran = true;
   f->list_used = std::min<uint64_t>(ctl_out[C_LIST_CURSOR * kCtlStride], f->list_words);
   f->list_ovf = acc_h[mbe::A_LIST_OVF];
+  memcpy(f->prof, acc_h, sizeof(acc_h));
   if (res->overflow)
     return set_error(GMBE_ERR_CAPACITY,
-                     "%llu task(s) exceeded a per-wave capacity (arena_kb=%lld, |L|<=%d words "
-                     "or queue); the result is incomplete",
-                     (unsigned long long)res->overflow, (long long)f->arena_kb, mbe::kMaxW * 64);
+                     "%llu task(s) exceeded a capacity (%llu list arena (arena_kb=%lld), %llu |L| > "
+                     "%d, %llu deferred-buffer); the result is incomplete",
+                     (unsigned long long)res->overflow, (unsigned long long)acc_h[mbe::A_OVF_ARENA],
+                     (long long)f->arena_kb, (unsigned long long)acc_h[mbe::A_OVF_WIDE],
+                     mbe::kMaxW * 64, (unsigned long long)acc_h[mbe::A_OVF_DEFER]);
   return GMBE_OK;
 }
 
@@ -514,6 +585,13 @@ int gmbe_finder_list(gmbe_finder* f, uint32_t* buf, size_t cap, size_t* used) {
     i += 2 + (size_t)nL + nR;
   }
   return GMBE_OK;
+}
+
+int gmbe_finder_stats(gmbe_finder* f, uint64_t* out, size_t n) {
+  if (!f) return set_error(GMBE_ERR_ARG, "null finder");
+  const int first = mbe::A_CYC_WAIT, avail = mbe::A_COUNT - mbe::A_CYC_WAIT;
+  for (size_t i = 0; out && i < n && i < (size_t)avail; ++i) out[i] = f->prof[first + i];
+  return avail;
 }
 
 void gmbe_finder_destroy(gmbe_finder* f) {

@@ -42,6 +42,8 @@ namespace mbe {
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr uint32_t kBranch = 0x80000000u;  // entry may still be pushed
 constexpr int kMaxPrefix = 8;              // task capacity (vertices)
+constexpr int kShards = 32;                // independent queue rings
+constexpr int kCtlStride = 32;             // u64 between control counters (256 B)
 constexpr int kMaxW = 16;                  // widest bitset (|L| <= 1024)
 constexpr int kMainW = 4;                  // widest bitset of the main kernel
 
@@ -62,17 +64,21 @@ struct DevGraph {
 // Counter slots of the result block (u64 each).
 enum Acc : int {
   A_MAXIMAL = 0, A_NONMAX, A_HSUM, A_HXOR, A_NODES, A_TASKS, A_SPLITS, A_OVERFLOW,
-  A_LIST_OVF, A_BALG, A_COUNT
+  A_LIST_OVF, A_BALG,
+  // profile (filled when Params::profile is set): cycles per phase, work sizes
+  A_CYC_WAIT, A_CYC_PREFIX, A_CYC_EXPAND, A_CYC_EXTRACT, A_CYC_BUILD, A_CYC_DFS, A_CYC_SPLIT,
+  A_EDGES, A_ENTRIES, A_OVF_ARENA, A_OVF_WIDE, A_OVF_DEFER, A_ENT_W1, A_NODES_W1, A_CYC_TAIL,
+  A_CYC_PASS, A_CYC_MERGE, A_CYC_COMPRESS, A_CYC_PUBLISH, A_PUBLISHES, A_MERGES, A_COUNT
 };
 
 struct Sched {
   unsigned long long* root_next;  // claim counter over top-level vertices
   unsigned long long* work_left;  // outstanding tasks (roots + children)
-  uint32_t* q;                    // ring of kEntryWords-word entries
+  uint32_t* q;                    // kShards rings of kEntryWords-word entries
   unsigned long long* qseq;       // per-slot sequence numbers
-  unsigned long long* qhead;      // next slot to pop
-  unsigned long long* qtail;      // next slot to reserve for a push
-  unsigned long long qcap;        // power of two
+  unsigned long long* qctl;       // per shard: head, tail (kCtlStride u64 apart)
+  unsigned long long* queued;     // entries pushed and not yet popped (a hint)
+  unsigned long long qcap;        // slots per shard, power of two
   unsigned long long n_roots;     // top-level vertices of this run (sample)
   uint32_t stride, phase;         // root r -> vertex nV-1-(r*stride+phase)
   const Task* root_tasks;         // if set, root r is root_tasks[r] instead
@@ -86,8 +92,8 @@ struct Scratch {
   uint64_t slab;     // bytes per wave
   // offsets inside a slab
   uint64_t o_cmap, o_bm0, o_bm1, o_L, o_Lp, o_T, o_kL, o_U, o_Ucnt, o_C, o_kC, o_R, o_fr,
-      o_arena;
-  uint64_t arena_bytes;
+      o_arena, o_ctx;
+  uint64_t arena_bytes, ctx_bytes;
   uint32_t nW0, nW1;
 };
 
@@ -100,20 +106,45 @@ struct Params {
   unsigned long long* list_cursor;
   unsigned long long list_cap;
   int bound_height, bound_size, prune, max_prefix, split;
+  int profile;               // accumulate per-phase cycle counts
+  int group;                 // group twin candidates (one-word bitsets)
+  int share;                 // split large tasks through shared contexts
+  int share_work;            // child entries x pushable entries that make a node shareable
+  int share_queue;           // share nodes only while fewer queue entries wait
+  int merge_min;             // merge twins of child lists at least this long
+  int merge_branch;          // ... with at least this many pushable entries
 };
 
 struct Counters {
   uint64_t maximal = 0, nonmax = 0, hsum = 0, hxor = 0, nodes = 0, tasks = 0, splits = 0,
            overflow = 0;
   uint64_t balg = 0;  // algorithmic bytes: 4 B per id of every list a node must read
+  uint64_t cyc[7] = {0, 0, 0, 0, 0, 0, 0};  // wait, prefix, expand, extract, build, dfs, split
+  uint64_t edges = 0, entries = 0, ovf_arena = 0, ovf_wide = 0, ovf_defer = 0;
+  uint64_t ent_w1 = 0, nodes_w1 = 0;  // the part of entries / pushes with one-word bitsets
+  uint64_t tail = 0;                   // cycles of the final wait for work
+  uint64_t c_pass = 0, c_merge = 0, c_compress = 0, c_publish = 0, publishes = 0, merges = 0;
 };
 
+__device__ __forceinline__ uint64_t cycles() { return __builtin_readcyclecounter(); }
+// Adds the cycles since `t0` to phase `ph` (when profiling) and restarts t0.
+__device__ __forceinline__ void phase(Counters& c, int on, int ph, uint64_t& t0) {
+  if (on) {
+    uint64_t t = cycles();
+    c.cyc[ph] += t - t0;
+    t0 = t;
+  }
+}
+
 struct Frame {
-  uint64_t sumR;
-  uint64_t rec;      // generic address of the first record
-  uint32_t len, cur, nbits, tier;  // tier 0 = LDS, 1 = HBM
-  uint32_t top;      // arena offset of this allocation (freed on pop)
-  uint32_t pad;
+  uint64_t sumR;     // key sum of R at this node
+  uint64_t rec;      // generic address of the first list entry
+  uint32_t len;      // entries
+  uint32_t cur;      // position of the next entry to push (kNone: exhausted)
+  uint32_t curidx;   // its candidate index
+  uint32_t nbits;    // |L| at this node
+  uint32_t tier;     // 0 = LDS arena, 1 = HBM arena
+  uint32_t top;      // arena offset of this list (freed on pop)
 };
 
 // ---------------------------------------------------------------------------
@@ -231,70 +262,106 @@ __device__ __forceinline__ void backoff(uint32_t& n) {
   n = n < 64 ? n * 2 : 64;
 }
 
-// Pop one entry into `word` (lane l < 32 holds word l).
-__device__ bool queue_pop(const Sched& s, uint32_t& word) {
-  uint32_t lane = lane_id();
+__device__ __forceinline__ unsigned long long* shard_head(const Sched& s, uint32_t sh) {
+  return s.qctl + (uint64_t)(2 * sh) * kCtlStride;
+}
+__device__ __forceinline__ unsigned long long* shard_tail(const Sched& s, uint32_t sh) {
+  return s.qctl + (uint64_t)(2 * sh + 1) * kCtlStride;
+}
+__device__ __forceinline__ uint64_t slot_of(const Sched& s, uint32_t sh, unsigned long long pos) {
+  return (uint64_t)sh * s.qcap + (pos & (s.qcap - 1));
+}
+
+// Pop one entry into `word` (lane l < 32 holds word l), scanning the shards
+// from `home`.
+__device__ bool queue_pop(const Sched& s, uint32_t& word, uint32_t home) {
+  const uint32_t lane = lane_id();
+  if ((long long)uni64(ld_dev(s.queued)) <= 0) return false;
   unsigned long long got = ~0ull;
+  uint32_t gsh = 0;
   if (lane == 0) {
-    unsigned long long h = ld_dev(s.qhead);
-    for (;;) {
-      unsigned long long tl = ld_dev(s.qtail);
-      if (h >= tl) break;
-      if (__hip_atomic_compare_exchange_strong(s.qhead, &h, h + 1, __ATOMIC_RELAXED,
-                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-        got = h;
-        break;
+    for (uint32_t i = 0; i < (uint32_t)kShards && got == ~0ull; ++i) {
+      const uint32_t sh = (home + i) & (kShards - 1);
+      unsigned long long h = ld_dev(shard_head(s, sh));
+      for (int tries = 0; tries < 4; ++tries) {
+        unsigned long long tl = ld_dev(shard_tail(s, sh));
+        if (h >= tl) break;
+        if (__hip_atomic_compare_exchange_strong(shard_head(s, sh), &h, h + 1, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          got = h;
+          gsh = sh;
+          break;
+        }
       }
-      __builtin_amdgcn_s_sleep(2);
     }
     if (got != ~0ull) {
-      unsigned long long slot = got & (s.qcap - 1);
+      __hip_atomic_fetch_add(s.queued, (unsigned long long)-1ll, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+      const uint64_t slot = slot_of(s, gsh, got);
       while (ld_dev(s.qseq + slot) != got + 1) __builtin_amdgcn_s_sleep(1);
     }
   }
   got = uni64(got);
   if (got == ~0ull) return false;
+  gsh = uni(gsh);
   wave_drain();
-  unsigned long long slot = got & (s.qcap - 1);
+  const uint64_t slot = slot_of(s, gsh, got);
   word = lane < (uint32_t)kEntryWords ? ld_dev(s.q + slot * kEntryWords + lane) : 0u;
   wave_drain();
   if (lane == 0) st_dev(s.qseq + slot, got + s.qcap);
   return true;
 }
 
-// Push the children prefix+[cand[i]], i < n, as ceil(n / kChunk) entries.
-// All or nothing: false (nothing pushed) when the ring lacks room.
-__device__ bool queue_push_children(const Sched& s, const Task& prefix, int k,
-                                    const uint32_t* cand, uint32_t n, uint32_t& n_entries) {
-  uint32_t lane = lane_id();
-  const uint32_t ne = (n + kChunk - 1) / kChunk;
-  n_entries = ne;
-  unsigned long long start = ~0ull;
+// Reserve ne consecutive positions in one shard, trying `home` first.
+// Returns false when every shard lacks room.
+__device__ bool queue_reserve(const Sched& s, uint32_t home, uint32_t ne, uint32_t& sh_out,
+                              unsigned long long& start) {
+  const uint32_t lane = lane_id();
+  unsigned long long st = ~0ull;
+  uint32_t sh_got = 0;
   if (lane == 0) {
-    unsigned long long tl = ld_dev(s.qtail);
-    for (;;) {
-      unsigned long long h = ld_dev(s.qhead);
-      if (tl + ne > h + s.qcap) break;
-      if (__hip_atomic_compare_exchange_strong(s.qtail, &tl, tl + ne, __ATOMIC_RELAXED,
-                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-        start = tl;
-        break;
+    for (uint32_t i = 0; i < (uint32_t)kShards && st == ~0ull; ++i) {
+      const uint32_t sh = (home + i) & (kShards - 1);
+      unsigned long long tl = ld_dev(shard_tail(s, sh));
+      for (;;) {
+        unsigned long long hd = ld_dev(shard_head(s, sh));
+        if (tl + ne > hd + s.qcap) break;
+        if (__hip_atomic_compare_exchange_strong(shard_tail(s, sh), &tl, tl + ne,
+                                                 __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT)) {
+          st = tl;
+          sh_got = sh;
+          break;
+        }
       }
     }
   }
-  start = uni64(start);
-  if (start == ~0ull) return false;
+  start = uni64(st);
+  sh_out = uni(sh_got);
+  return start != ~0ull;
+}
+
+// Push the children prefix+[cand[i]], i < n, as ceil(n / kChunk) entries.
+// All or nothing: false (nothing pushed) when the queue lacks room.
+__device__ bool queue_push_children(const Sched& s, const Task& prefix, int k,
+                                    const uint32_t* cand, uint32_t n, uint32_t& n_entries,
+                                    uint32_t home) {
+  uint32_t lane = lane_id();
+  const uint32_t ne = (n + kChunk - 1) / kChunk;
+  n_entries = ne;
+  uint32_t sh = 0;
+  unsigned long long start = 0;
+  if (!queue_reserve(s, home, ne, sh, start)) return false;
   // two entries per pass: lanes 0-31 and 32-63
   for (uint32_t e0 = 0; e0 < ne; e0 += 2) {
     uint32_t e = e0 + (lane >> 5), w = lane & 31;
-    if (e < ne) {
-      unsigned long long pos = start + e, slot = pos & (s.qcap - 1);
-      if (w == 0)
-        while (ld_dev(s.qseq + slot) != pos) __builtin_amdgcn_s_sleep(1);
+    if (e < ne && w == 0) {
+      const unsigned long long pos = start + e;
+      while (ld_dev(s.qseq + slot_of(s, sh, pos)) != pos) __builtin_amdgcn_s_sleep(1);
     }
     wave_drain();
     if (e < ne) {
-      unsigned long long slot = (start + e) & (s.qcap - 1);
+      const uint64_t slot = slot_of(s, sh, start + e);
       uint32_t cnt = n - e * kChunk < (uint32_t)kChunk ? n - e * kChunk : (uint32_t)kChunk;
       uint32_t val;
       if (w == 0) val = (uint32_t)k | (cnt << 8);
@@ -303,9 +370,12 @@ __device__ bool queue_push_children(const Sched& s, const Task& prefix, int k,
       st_dev(s.q + slot * kEntryWords + w, val);
     }
     wave_drain();  // the words are in place before the slot is published
-    if (e < ne && w == 0) st_dev(s.qseq + ((start + e) & (s.qcap - 1)), start + e + 1);
+    if (e < ne && w == 0) st_dev(s.qseq + slot_of(s, sh, start + e), start + e + 1);
   }
   wave_drain();
+  if (lane == 0)
+    __hip_atomic_fetch_add(s.queued, (unsigned long long)ne, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
   return true;
 }
 
@@ -314,12 +384,12 @@ __device__ bool queue_push_children(const Sched& s, const Task& prefix, int k,
 // every unit has finished (the gc/lc/tc protocol of
 // src/IterFinderGpu.cu:978-986,1020-1034 collapsed into one outstanding-work
 // counter), 1 for an entry (in `word`), 2 for a root task (in `t`, `k`).
-__device__ int next_work(const Params& P, uint32_t& word, Task& t, int& k) {
+__device__ int next_work(const Params& P, uint32_t& word, Task& t, int& k, uint32_t home) {
   const Sched& s = P.s;
   uint32_t lane = lane_id();
   uint32_t nap = 1;
   for (;;) {
-    if (queue_pop(s, word)) return 1;
+    if (queue_pop(s, word, home)) return 1;
     unsigned long long r = ~0ull;
     if (lane == 0 && ld_dev(s.root_next) < s.n_roots) {
       r = __hip_atomic_fetch_add(s.root_next, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -432,17 +502,33 @@ __device__ uint64_t list_reserve(const Params& P, uint64_t words) {
 }
 
 // ---------------------------------------------------------------------------
-// the DFS over one task (W = bitset words per entry)
+// the stack machine over one task (W = bitset words per entry)
+//
+// A list entry is a group of candidates whose neighbourhoods inside the
+// node's L are identical.  Such twins behave identically under every later
+// push: the first one that is pushed absorbs the others into R' and PRUNE_EN
+// then prunes them, and a twin before it makes it non-canonical.  So a group
+// is fully described by
+//   minidx  its smallest member index,
+//   rep     the member that may still be pushed (kNone when none may),
+//   ksum    the members' hash-key sum,
+//   a       the bitset,
+// and one entry per group gives the reference's pushes, verdicts and prunes.
+// Entries that become twins inside a child's smaller L are merged again
+// (minidx and rep by minimum, ksum by sum), so a node's list never holds more
+// than 2^|L| - 2 entries.  Entries of a list are in no particular order:
+// pushes go by increasing rep.
 
 template <int W>
 struct Rec {
-  uint32_t idx;  // candidate index (into C) | kBranch
-  uint32_t pad;
+  uint32_t minidx;
+  uint32_t rep;
+  uint64_t ksum;  // hash-key sum of the group's members
   uint64_t a[W];
 };
 
-// Allocate `bytes` for a child list: LDS first, then HBM.  Returns the
-// generic address or 0 when neither arena has room.
+// Allocate `bytes` for a list: LDS first, then HBM.  Returns the generic
+// address or 0 when neither arena has room.
 __device__ __forceinline__ uint64_t arena_take(const WaveMem& m, uint32_t& lds_top,
                                                uint64_t& hbm_top, uint64_t bytes,
                                                uint32_t& tier, uint64_t& top_before) {
@@ -459,22 +545,520 @@ __device__ __forceinline__ uint64_t arena_take(const WaveMem& m, uint32_t& lds_t
   return 0;
 }
 
+__device__ __forceinline__ uint64_t wave_min64(uint64_t x) {
+  for (int s = 32; s > 0; s >>= 1) {
+    uint32_t lo = shfl_xor32((uint32_t)x, s), hi = shfl_xor32((uint32_t)(x >> 32), s);
+    uint64_t y = ((uint64_t)hi << 32) | lo;
+    x = y < x ? y : x;
+  }
+  return x;
+}
+
+// Task-level arrays a stack machine reads: hash keys of L (by L index) and
+// candidate vertex ids.  When REMOTE, they belong to another wave's shared
+// context and are read with device-scope loads.
+struct TaskArrays {
+  const uint64_t* kL;
+  const uint32_t* C;
+};
+
+template <bool REMOTE, class T>
+__device__ __forceinline__ T rd(const T* p) {
+  if constexpr (REMOTE) return ld_dev(p);
+  else return *p;
+}
+
+struct PassOut {
+  uint32_t cbranch;     // pushable entries of the child list
+  bool nonmax;
+  uint64_t absorbed;    // key sum of the groups joining R'
+  uint64_t best_next;   // (rep << 32 | position) of the next push in the parent
+  uint64_t best_child;  // same, for the child's first push
+  uint32_t clen;
+};
+
+// One push: x (candidate index xidx, bitset ax) against the parent's list
+// `recs`, writing the child list to `crec`.  LIST_REMOTE: the parent list is
+// another wave's shared context (read-only: no prune writes).
+template <int W, bool LIST_REMOTE>
+__device__ PassOut push_pass(Rec<W>* recs, uint32_t len, const uint64_t* ax, uint32_t xidx,
+                             Rec<W>* crec, bool prune) {
+  const uint32_t lane = lane_id();
+  bool nonmax = false;
+  uint64_t absorbed = 0, best_next = ~0ull, best_child = ~0ull;
+  uint32_t clen = 0, cbranch = 0;
+  for (uint32_t base = 0; base < len; base += kWave) {
+    const uint32_t k = base + lane;
+    const bool valid = k < len;
+    uint32_t mi = kNone, rep = kNone;
+    uint64_t me[W];
+    bool sub = true, zero = true, inside = true;
+#pragma unroll
+    for (int w = 0; w < W; ++w) me[w] = 0;
+    if (valid) {
+      mi = rd<LIST_REMOTE>(&recs[k].minidx);
+      rep = rd<LIST_REMOTE>(&recs[k].rep);
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        uint64_t ae = rd<LIST_REMOTE>(&recs[k].a[w]);
+        me[w] = ae & ax[w];
+        sub = sub && (me[w] == ax[w]);
+        zero = zero && (me[w] == 0);
+        inside = inside && ((ae & ~ax[w]) == 0);
+      }
+    }
+    uint64_t ks = 0;
+    if (valid) ks = rd<LIST_REMOTE>(&recs[k].ksum);
+    if (valid && sub) {
+      if (mi < xidx) nonmax = true;  // a twin or superset before x
+      else absorbed += ks;           // joins R'
+    }
+    const bool br = rep != kNone && rep > xidx;
+    const bool pruned = prune && valid && br && inside;
+    if constexpr (!LIST_REMOTE) {
+      if (pruned) recs[k].rep = kNone;
+    }
+    const bool keep = valid && !sub && !zero;
+    const uint32_t crep = br ? rep : kNone;  // branchability before this push's prune
+    const uint64_t km = ballot(keep);
+    const uint32_t pos = clen + lanes_below(km);
+    if (keep) {
+      crec[pos].minidx = mi;
+      crec[pos].rep = crep;
+      crec[pos].ksum = ks;
+#pragma unroll
+      for (int w = 0; w < W; ++w) crec[pos].a[w] = me[w];
+      if (crep != kNone) {
+        uint64_t c = ((uint64_t)crep << 32) | pos;
+        best_child = c < best_child ? c : best_child;
+      }
+    }
+    if (valid && br && !pruned) {
+      uint64_t c = ((uint64_t)rep << 32) | k;
+      best_next = c < best_next ? c : best_next;
+    }
+    clen += __popcll(km);
+    cbranch += __popcll(ballot(keep && crep != kNone));
+  }
+  PassOut o;
+  o.cbranch = cbranch;
+  o.nonmax = ballot(nonmax) != 0;
+  o.absorbed = uni64(wave_sum64(absorbed));
+  o.best_next = uni64(wave_min64(best_next));
+  o.best_child = uni64(wave_min64(best_child));
+  o.clen = clen;
+  return o;
+}
+
+template <int W, bool KL_R>
+__device__ __forceinline__ uint64_t key_sum_of_bits(const uint64_t* ax, const TaskArrays& ta) {
+  const uint32_t lane = lane_id();
+  uint64_t sl = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w)
+    if ((ax[w] >> lane) & 1ull) sl += rd<KL_R>(ta.kL + w * 64 + lane);
+  return uni64(wave_sum64(sl));
+}
+
+// Re-index the bitsets of a list onto the set bits of ax (at most 64): entry
+// bit t of the output is the input's bit at the t-th set position of ax.
 template <int W>
-__device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC,
-                    uint32_t first, uint64_t sumR0, uint64_t root_rec, uint32_t root_tier,
-                    uint64_t root_top, uint32_t& lds_top, uint64_t& hbm_top, Counters& ctr,
-                    uint32_t nR0) {
+__device__ void compress_list(const Rec<W>* src, uint32_t len, const uint64_t* ax, Rec<1>* dst) {
+  const uint32_t lane = lane_id();
+  for (uint32_t i = lane; i < len; i += kWave) {
+    uint64_t out = 0;
+    uint32_t t = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      uint64_t mw = ax[w];
+      const uint64_t aw = src[i].a[w];
+      while (mw) {
+        const uint32_t b = __builtin_ctzll(mw);
+        mw &= mw - 1;
+        out |= ((aw >> b) & 1ull) << t;
+        ++t;
+      }
+    }
+    dst[i].minidx = src[i].minidx;
+    dst[i].rep = src[i].rep;
+    dst[i].ksum = src[i].ksum;
+    dst[i].a[0] = out;
+  }
+  wave_mem_wait();
+}
+
+// ---------------------------------------------------------------------------
+// shared task contexts
+//
+// A task the split rule marks as large publishes its root list once, in the
+// context area of its wave's slab, and pushes one work item (entry position,
+// candidate) per pushable group.  Any wave runs an item: one push pass over
+// the shared list (device-scope loads), then the stack machine on the child
+// list in its own arenas.  This replaces the reference's re-computation of
+// every child in LargeToTiny (src/IterFinderGpu.cu:889-910) and in the tiny
+// tasks, whose cost here was dominated by re-reading hub neighbourhoods.
+
+struct CtxHdr {
+  uint64_t sumR0;           // key sum of R at the shared node
+  unsigned long long refs;  // items not yet finished
+  uint32_t nL, nC, len, W;  // task |L| and |C|, list length, bitset words
+  uint32_t nbits;           // |L| at the shared node
+  uint32_t pad[9];
+};
+
+__device__ __forceinline__ CtxHdr* ctx_hdr(const Params& P, uint64_t wave) {
+  return (CtxHdr*)(P.w.base + wave * P.w.slab + P.w.o_ctx);
+}
+// context layout after the header: list (len recs), kL (nL), C (nC)
+template <int W>
+__device__ __forceinline__ void ctx_arrays(CtxHdr* h, uint32_t nL, uint32_t nC, uint32_t len,
+                                           Rec<W>*& recs, TaskArrays& ta) {
+  char* p = (char*)h + sizeof(CtxHdr);
+  recs = (Rec<W>*)p;
+  p += (uint64_t)len * sizeof(Rec<W>);
+  ta.kL = (const uint64_t*)p;
+  p += (uint64_t)nL * 8;
+  ta.C = (const uint32_t*)p;
+}
+template <int W>
+__device__ __forceinline__ uint64_t ctx_bytes(uint32_t nL, uint32_t nC, uint32_t len) {
+  return sizeof(CtxHdr) + (uint64_t)len * sizeof(Rec<W>) + (uint64_t)nC * 4 + (uint64_t)nL * 8;
+}
+
+constexpr uint32_t kItemKind = 0x80000000u;
+constexpr int kItemsPerEntry = 15;  // (position, candidate) pairs after hdr, owner
+
+// Publish a node's list `root` (len entries, in this wave's arenas) as this
+// wave's context and push one item per pushable entry.  The task arrays
+// `src` (local or remote) are copied so that the context is self-contained.
+// Returns false (nothing published) when the queue lacks room or the node has
+// nothing to push.
+template <int W, bool KL_R, bool C_R>
+__device__ bool publish_list(const Params& P, uint64_t gw, const Rec<W>* root, uint32_t len,
+                             uint32_t nL, uint32_t nC, uint32_t nbits, uint64_t sumR0,
+                             const TaskArrays& src) {
+  const uint32_t lane = lane_id();
+  const Sched& s = P.s;
+  CtxHdr* h = ctx_hdr(P, gw);
+  Rec<W>* recs;
+  TaskArrays ta;
+  ctx_arrays<W>(h, nL, nC, len, recs, ta);
+  // items: pushable groups, in the order of the list
+  uint32_t n_items = 0;
+  for (uint32_t base = 0; base < len; base += kWave) {
+    uint32_t i = base + lane;
+    bool it = i < len && root[i].rep != kNone;
+    n_items += __popcll(ballot(it));
+  }
+  if (n_items == 0) return false;
+  const uint32_t ne = (n_items + kItemsPerEntry - 1) / kItemsPerEntry;
+  // copy the context with device-scope stores
+  for (uint32_t i = lane; i < len; i += kWave) {
+    st_dev(&recs[i].minidx, root[i].minidx);
+    st_dev(&recs[i].rep, root[i].rep);
+    st_dev(&recs[i].ksum, root[i].ksum);
+#pragma unroll
+    for (int w = 0; w < W; ++w) st_dev(&recs[i].a[w], root[i].a[w]);
+  }
+  for (uint32_t i = lane; i < nC; i += kWave)
+    st_dev((uint32_t*)ta.C + i, rd<C_R>(src.C + i));
+  for (uint32_t j = lane; j < nL; j += kWave)
+    st_dev((uint64_t*)ta.kL + j, rd<KL_R>(src.kL + j));
+  if (lane == 0) {
+    st_dev(&h->sumR0, sumR0);
+    st_dev(&h->nbits, nbits);
+    st_dev(&h->nL, nL);
+    st_dev(&h->nC, nC);
+    st_dev(&h->len, len);
+    st_dev(&h->W, (uint32_t)W);
+    st_dev(&h->refs, (unsigned long long)n_items);
+  }
+  wave_drain();
+  // reserve ne entries in one shard
+  uint32_t sh = 0;
+  unsigned long long start = 0;
+  if (!queue_reserve(s, (uint32_t)(gw & (kShards - 1)), ne, sh, start)) {
+    if (lane == 0) st_dev(&h->refs, 0ull);
+    return false;
+  }
+  if (lane == 0)
+    __hip_atomic_fetch_add(s.work_left, (unsigned long long)ne, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+  // fill the entries: item j goes to entry j / 15, slot j % 15
+  uint32_t j0 = 0;
+  for (uint32_t base = 0; base < len; base += kWave) {
+    uint32_t i = base + lane;
+    bool it = i < len && root[i].rep != kNone;
+    uint64_t bm = ballot(it);
+    if (it) {
+      uint32_t j = j0 + lanes_below(bm);
+      const unsigned long long pos = start + j / kItemsPerEntry;
+      const uint64_t slot = slot_of(s, sh, pos);
+      uint32_t* e = s.q + slot * kEntryWords;
+      if (j % kItemsPerEntry == 0) {
+        // first item of an entry: wait until the slot is free, write the header
+        while (ld_dev(s.qseq + slot) != pos) __builtin_amdgcn_s_sleep(1);
+        uint32_t cnt = n_items - j < (uint32_t)kItemsPerEntry ? n_items - j : kItemsPerEntry;
+        st_dev(e + 0, kItemKind | cnt);
+        st_dev(e + 1, (uint32_t)gw);
+      }
+    }
+    wave_drain();
+    if (it) {
+      uint32_t j = j0 + lanes_below(bm);
+      const uint64_t slot = slot_of(s, sh, start + j / kItemsPerEntry);
+      uint32_t* e = s.q + slot * kEntryWords;
+      st_dev(e + 2 + 2 * (j % kItemsPerEntry), i);
+      st_dev(e + 3 + 2 * (j % kItemsPerEntry), root[i].rep);
+    }
+    j0 += __popcll(bm);
+  }
+  wave_drain();
+  for (uint32_t e = lane; e < ne; e += kWave) st_dev(s.qseq + slot_of(s, sh, start + e), start + e + 1);
+  wave_drain();
+  if (lane == 0)
+    __hip_atomic_fetch_add(s.queued, (unsigned long long)ne, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
+// The stack machine from frame F (a list in this wave's arenas).
+// Merge the entries of a list (len entries at crec) whose bitsets are
+// identical: minidx and rep by minimum, ksum by sum.  Open addressing in a
+// table taken from the arenas (LDS first); a slot's identity word packs a
+// 32-bit hash of the bitset with the index of its first entry, set by one CAS,
+// and a probe that meets the same hash compares the full bitsets, so no lane
+// ever waits for another.  Returns the new length and recomputes the first
+// push and the pushable count; leaves the list unchanged (returns len) when
+// the arenas lack room.
+template <int W>
+__device__ uint32_t merge_twins(const WaveMem& m, Rec<W>* crec, uint32_t len, uint32_t& lds_top,
+                                uint64_t& hbm_top, uint64_t& best_child, uint32_t& cbranch) {
+  const uint32_t lane = lane_id();
+  struct Slot {
+    unsigned long long id;  // hash << 32 | (first entry + 1); 0 = empty
+    uint32_t mi, rep;
+    uint64_t ksum;
+  };
+  uint32_t T = 64;
+  while (T < 2 * len) T <<= 1;
+  const uint32_t save_l = lds_top;
+  const uint64_t save_h = hbm_top;
+  uint32_t tier = 1, otier = 1;
+  uint64_t tb = 0, ob = 0;
+  uint64_t taddr = arena_take(m, lds_top, hbm_top, (uint64_t)T * sizeof(Slot), tier, tb);
+  if (taddr == 0) return len;
+  if (tier == 0) lds_top = (uint32_t)(tb + (uint64_t)T * sizeof(Slot));
+  else hbm_top = tb + (uint64_t)T * sizeof(Slot);
+  uint64_t oaddr = arena_take(m, lds_top, hbm_top, (uint64_t)len * sizeof(Rec<W>), otier, ob);
+  lds_top = save_l;
+  hbm_top = save_h;
+  if (oaddr == 0) return len;
+  Slot* tab = (Slot*)taddr;
+  Rec<W>* out = (Rec<W>*)oaddr;
+  for (uint32_t i = lane; i < T; i += kWave) {
+    tab[i].id = 0;
+    tab[i].mi = kNone;
+    tab[i].rep = kNone;
+    tab[i].ksum = 0;
+  }
+  wave_mem_wait();
+  for (uint32_t i = lane; i < len; i += kWave) {
+    uint64_t key[W];
+    uint64_t hk = 0x2545F4914F6CDD1Dull;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      key[w] = crec[i].a[w];
+      hk = mix64(hk ^ key[w]);
+    }
+    const unsigned long long me = ((hk >> 32) << 32) | (unsigned long long)(i + 1);
+    uint32_t h = (uint32_t)hk & (T - 1);
+    for (;;) {
+      unsigned long long old = 0;
+      if (__hip_atomic_compare_exchange_strong(&tab[h].id, &old, me, __ATOMIC_RELAXED,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+        break;  // first entry of this bitset
+      if ((old >> 32) == (hk >> 32)) {
+        const uint32_t f = (uint32_t)(old & 0xffffffffu) - 1u;
+        bool same = true;
+#pragma unroll
+        for (int w = 0; w < W; ++w) same = same && (crec[f].a[w] == key[w]);
+        if (same) break;
+      }
+      h = (h + 1) & (T - 1);
+    }
+    __hip_atomic_fetch_min(&tab[h].mi, crec[i].minidx, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+    const uint32_t rp = crec[i].rep;
+    if (rp != kNone)
+      __hip_atomic_fetch_min(&tab[h].rep, rp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    add_wg(&tab[h].ksum, crec[i].ksum);
+  }
+  wave_mem_wait();
+  uint32_t n = 0, nbr = 0;
+  uint64_t best = ~0ull;
+  for (uint32_t base = 0; base < T; base += kWave) {
+    const uint32_t i = base + lane;
+    const unsigned long long id = i < T ? ld_wg(&tab[i].id) : 0ull;
+    const uint64_t bm = ballot(id != 0);
+    uint32_t rp = kNone;
+    if (id != 0) {
+      const uint32_t pos = n + lanes_below(bm), f = (uint32_t)(id & 0xffffffffu) - 1u;
+      rp = ld_wg(&tab[i].rep);
+      out[pos].minidx = ld_wg(&tab[i].mi);
+      out[pos].rep = rp;
+      out[pos].ksum = ld_wg(&tab[i].ksum);
+#pragma unroll
+      for (int w = 0; w < W; ++w) out[pos].a[w] = crec[f].a[w];
+      if (rp != kNone) {
+        const uint64_t c = ((uint64_t)rp << 32) | pos;
+        best = c < best ? c : best;
+      }
+    }
+    nbr += __popcll(ballot(rp != kNone));
+    n += __popcll(bm);
+  }
+  wave_mem_wait();
+  for (uint32_t i = lane; i < n; i += kWave) crec[i] = out[i];
+  wave_mem_wait();
+  best_child = uni64(wave_min64(best));
+  cbranch = nbr;
+  return n;
+}
+
+// Whether sharing a node is worthwhile now: this wave's context area is free
+// and large enough, the top-level vertices are all claimed and the queue is
+// short.
+__device__ __forceinline__ bool share_wanted(const Params& P, uint64_t gw, uint32_t nL, uint32_t nC,
+                                             uint32_t len, int W) {
+  const uint64_t bytes = sizeof(CtxHdr) + (uint64_t)len * (16 + 8 * W) + (uint64_t)nC * 4 +
+                         (uint64_t)nL * 8;
+  if (bytes > P.w.ctx_bytes) return false;
+  if (uni64(ld_dev(&ctx_hdr(P, gw)->refs)) != 0ull) return false;
+  if (uni64(ld_dev(P.s.root_next)) < P.s.n_roots) return false;
+  const long long depth = (long long)uni64(ld_dev(P.s.queued));
+  return depth < (long long)P.share_queue;
+}
+
+template <int W, bool KL_R, bool C_R>
+__device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC, Frame F,
+                    uint32_t& lds_top, uint64_t& hbm_top, Counters& ctr, const TaskArrays& ta,
+                    uint64_t root_rec, uint32_t root_len, uint32_t nR0, uint64_t gw, Frame* fr);
+
+// What to do with a child node (list at crec, committed in the arenas): run it
+// with re-indexed one-word bitsets when its L fits one word and the task's
+// does not, merge twins, share it with other waves when it is large and work
+// is scarce.  Returns true when the caller should descend into (crec, clen)
+// itself; false when the node has been handled here.
+template <int W, bool KL_R, bool C_R>
+__device__ bool child_node(const Params& P, const WaveMem& m, uint64_t gw, Counters& ctr,
+                           uint32_t nL, uint32_t nC, const TaskArrays& ta, Rec<W>* crec,
+                           uint32_t& clen, uint64_t& best_child, uint32_t& cbranch, uint32_t nb,
+                           const uint64_t* ax, uint64_t sumR, uint32_t& lds_top,
+                           uint64_t& hbm_top, Frame* fr_next) {
+  const uint32_t lane = lane_id();
+  if constexpr (W > 1) {
+    if (nb <= 64 && !P.list) {
+      const uint32_t sl = lds_top;
+      const uint64_t sh = hbm_top;
+      uint32_t kt = 1, dt = 1;
+      uint64_t ktop = 0, dtop = 0;
+      uint64_t kaddr = arena_take(m, lds_top, hbm_top, 64 * 8, kt, ktop);
+      if (kaddr) {
+        if (kt == 0) lds_top = (uint32_t)(ktop + 512); else hbm_top = ktop + 512;
+      }
+      uint64_t daddr = kaddr ? arena_take(m, lds_top, hbm_top, (uint64_t)clen * sizeof(Rec<1>),
+                                          dt, dtop) : 0;
+      if (daddr) {
+        if (dt == 0) lds_top = (uint32_t)(dtop + (uint64_t)clen * sizeof(Rec<1>));
+        else hbm_top = dtop + (uint64_t)clen * sizeof(Rec<1>);
+        uint64_t* kLc = (uint64_t*)kaddr;
+        Rec<1>* dst = (Rec<1>*)daddr;
+        // keys of the child's L in bit order
+        {
+          uint32_t t = 0;
+          for (int w = 0; w < W; ++w) {
+            const uint64_t mw = ax[w];
+            const bool b = (mw >> lane) & 1ull;
+            const uint32_t pos = t + __popcll(mw & ((1ull << lane) - 1ull));
+            if (b) kLc[pos] = rd<KL_R>(ta.kL + w * 64 + lane);
+            t += __popcll(mw);
+          }
+        }
+        uint64_t tc0 = P.profile ? cycles() : 0;
+        compress_list<W>(crec, clen, ax, dst);
+        if (P.profile) ctr.c_compress += cycles() - tc0;
+        uint32_t dlen = clen, dcb = cbranch;
+        uint64_t dbest = best_child;
+        if (P.group && dlen >= (uint32_t)P.merge_min && dcb >= (uint32_t)P.merge_branch) {
+          uint64_t tm = P.profile ? cycles() : 0;
+          dlen = merge_twins<1>(m, dst, dlen, lds_top, hbm_top, dbest, dcb);
+          if (P.profile) ctr.c_merge += cycles() - tm;
+          ctr.merges++;
+        }
+        TaskArrays tc{kLc, ta.C};
+        bool shared = false;
+        if (P.share && (uint64_t)dlen * dcb >= (uint64_t)P.share_work &&
+            share_wanted(P, gw, nb, nC, dlen, 1)) {
+          uint64_t tq = P.profile ? cycles() : 0;
+          shared = publish_list<1, false, C_R>(P, gw, dst, dlen, nb, nC, nb, sumR, tc);
+          if (P.profile) ctr.c_publish += cycles() - tq;
+        }
+        if (shared) {
+          ctr.splits++;
+          ctr.publishes++;
+        } else {
+          if (dt == 0) lds_top = (uint32_t)(dtop + (uint64_t)dlen * sizeof(Rec<1>));
+          else hbm_top = dtop + (uint64_t)dlen * sizeof(Rec<1>);
+          Frame G;
+          G.sumR = sumR;
+          G.rec = daddr;
+          G.len = dlen;
+          G.cur = (uint32_t)dbest;
+          G.curidx = (uint32_t)(dbest >> 32);
+          G.nbits = nb;
+          G.tier = dt;
+          G.top = (uint32_t)dtop;
+          dfs<1, false, C_R>(P, m, nb, nC, G, lds_top, hbm_top, ctr, tc, 0, 0, 0, gw, fr_next);
+        }
+        lds_top = sl;
+        hbm_top = sh;
+        return false;
+      }
+      lds_top = sl;
+      hbm_top = sh;
+    }
+  }
+  if (P.group && !P.list && clen >= (uint32_t)P.merge_min &&
+      cbranch >= (uint32_t)P.merge_branch) {
+    uint64_t tm = P.profile ? cycles() : 0;
+    clen = merge_twins<W>(m, crec, clen, lds_top, hbm_top, best_child, cbranch);
+    if (P.profile) ctr.c_merge += cycles() - tm;
+    ctr.merges++;
+  }
+  if (P.share && !P.list && (uint64_t)clen * cbranch >= (uint64_t)P.share_work &&
+      share_wanted(P, gw, nL, nC, clen, W)) {
+    uint64_t tq = P.profile ? cycles() : 0;
+    bool ok = publish_list<W, KL_R, C_R>(P, gw, crec, clen, nL, nC, nb, sumR, ta);
+    if (P.profile) ctr.c_publish += cycles() - tq;
+    if (ok) {
+      ctr.splits++;
+      ctr.publishes++;
+      return false;
+    }
+  }
+  return true;
+}
+
+// The stack machine from frame F (a list in this wave's arenas); saved
+// frames go to fr[0..).
+template <int W, bool KL_R, bool C_R>
+__device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC, Frame F,
+                    uint32_t& lds_top, uint64_t& hbm_top, Counters& ctr, const TaskArrays& ta,
+                    uint64_t root_rec, uint32_t root_len, uint32_t nR0, uint64_t gw, Frame* fr) {
   typedef Rec<W> R_t;
   const uint32_t lane = lane_id();
   const bool prune = P.prune != 0;
-  Frame F;
-  F.sumR = sumR0;
-  F.rec = root_rec;
-  F.len = nC;
-  F.cur = first;
-  F.nbits = nL;
-  F.tier = root_tier;
-  F.top = (uint32_t)root_top;
   uint32_t depth = 0;
   for (;;) {
     if (F.cur == kNone) {
@@ -483,21 +1067,22 @@ __device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC,
       if (depth == 0) break;
       --depth;
       wave_mem_wait();
-      F = m.fr[depth];
+      F = fr[depth];
       F.sumR = uni64(F.sumR);
       F.rec = uni64(F.rec);
       F.len = uni(F.len);
       F.cur = uni(F.cur);
+      F.curidx = uni(F.curidx);
       F.nbits = uni(F.nbits);
       F.tier = uni(F.tier);
       F.top = uni(F.top);
       continue;
     }
     R_t* recs = (R_t*)F.rec;
-    const uint32_t xpos = F.cur;
+    const uint32_t xpos = F.cur, xidx = F.curidx;
     {
       // B(n) = 4 (deg(x) + |L_parent| + |C|)
-      uint32_t xc = P.g.nV ? m.C[uni(recs[xpos].idx) & ~kBranch] : 0;
+      uint32_t xc = rd<C_R>(ta.C + xidx);
       uint64_t dx = (uint64_t)(P.g.v_off[xc + 1] - P.g.v_off[xc]);
       ctr.balg += 4ull * (dx + F.nbits + nC);
     }
@@ -508,145 +1093,106 @@ __device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC,
       ax[w] = uni64(recs[xpos].a[w]);
       nb += __popcll(ax[w]);
     }
-    // speculative child list
+    // speculative child list (at most as long as the parent's)
     uint32_t ctier = 1;
     uint64_t ctop = 0;
-    uint64_t need = (uint64_t)F.len * sizeof(R_t);
-    uint64_t crec_addr = arena_take(m, lds_top, hbm_top, need, ctier, ctop);
+    uint64_t crec_addr =
+        arena_take(m, lds_top, hbm_top, (uint64_t)F.len * sizeof(R_t), ctier, ctop);
     if (crec_addr == 0) {
       // no room for the child list: report and abandon the task
       ctr.overflow++;
-      if (F.tier == 0) lds_top = F.top; else hbm_top = F.top;
-      while (depth > 0) {
-        --depth;
-        Frame G = m.fr[depth];
-        if (uni(G.tier) == 0) lds_top = uni(G.top); else hbm_top = uni(G.top);
-      }
+      ctr.ovf_arena++;
       return;
     }
     R_t* crec = (R_t*)crec_addr;
-    bool nonmax = false;
-    uint64_t absorbed = 0;  // per-lane key sum of entries joining R'
-    uint32_t clen = 0, cfirst = kNone, next = kNone;
-    for (uint32_t base = 0; base < F.len; base += kWave) {
-      uint32_t k = base + lane;
-      bool valid = k < F.len;
-      uint32_t f = 0;
-      uint64_t ae[W], me[W];
-      bool sub = true, zero = true, inside = true;
-#pragma unroll
-      for (int w = 0; w < W; ++w) {
-        ae[w] = 0;
-        me[w] = 0;
-      }
-      if (valid) {
-        f = recs[k].idx;
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-          ae[w] = recs[k].a[w];
-          me[w] = ae[w] & ax[w];
-          sub = sub && (me[w] == ax[w]);
-          zero = zero && (me[w] == 0);
-          inside = inside && ((ae[w] & ~ax[w]) == 0);
-        }
-      }
-      const bool before = k < xpos, after = k > xpos;
-      const bool br = (f & kBranch) != 0;
-      const uint32_t e = f & ~kBranch;
-      if (valid && sub && before) nonmax = true;
-      if (valid && sub && !before) absorbed += m.kC[e];
-      const bool pruned = prune && valid && after && br && inside;
-      if (pruned) recs[k].idx = e;
-      const bool keep = valid && !sub && !zero;
-      const bool cbr = br && after;
-      uint64_t km = ballot(keep);
-      uint32_t pos = clen + lanes_below(km);
-      if (keep) {
-        crec[pos].idx = e | (cbr ? kBranch : 0u);
-#pragma unroll
-        for (int w = 0; w < W; ++w) crec[pos].a[w] = me[w];
-      }
-      if (cfirst == kNone) {
-        uint64_t bm = ballot(keep && cbr);
-        if (bm) cfirst = clen + __popcll(km & ((1ull << first_lane(bm)) - 1ull));
-      }
-      clen += __popcll(km);
-      if (next == kNone) {
-        uint64_t nm = ballot(valid && after && br && !pruned);
-        if (nm) next = base + first_lane(nm);
-      }
-    }
-    nonmax = ballot(nonmax) != 0;
+    uint64_t tp = P.profile ? cycles() : 0;
+    PassOut o = push_pass<W, false>(recs, F.len, ax, xidx, crec, prune);
+    if (P.profile) ctr.c_pass += cycles() - tp;
     ctr.nodes++;
-    F.cur = next;
-    if (nonmax) {
+    ctr.entries += F.len;
+    if (W == 1) {
+      ctr.ent_w1 += F.len;
+      ctr.nodes_w1++;
+    }
+    F.cur = o.best_next == ~0ull ? kNone : (uint32_t)o.best_next;
+    F.curidx = (uint32_t)(o.best_next >> 32);
+    if (o.nonmax) {
       ctr.nonmax++;
       continue;
     }
-    // L' key sum: bits of ax over kL
-    uint64_t sl = 0;
-#pragma unroll
-    for (int w = 0; w < W; ++w)
-      if ((ax[w] >> lane) & 1ull) sl += m.kL[w * 64 + lane];
-    uint64_t sumL = uni64(wave_sum64(sl));
-    uint64_t sumR = F.sumR + uni64(wave_sum64(absorbed));
+    const uint64_t sumL = key_sum_of_bits<W, KL_R>(ax, ta);
+    const uint64_t sumR = F.sumR + o.absorbed;
     add_biclique(ctr, biclique_hash(sumL, sumR, P.g.swapped));
-    if (P.list) {
-      // R' = R0 ∪ {C[i] : a_i ⊇ L'} over the root list
-      const R_t* root = (const R_t*)root_rec;
-      uint32_t nr = nR0;
-      for (uint32_t base = 0; base < nC; base += kWave) {
-        uint32_t i = base + lane;
-        bool s = false;
-        if (i < nC) {
-          s = true;
-#pragma unroll
-          for (int w = 0; w < W; ++w) s = s && ((root[i].a[w] & ax[w]) == ax[w]);
-        }
-        nr += __popcll(ballot(s));
-      }
-      uint64_t at = list_reserve(P, 2ull + nb + nr);
-      if (at != ~0ull) {
-        uint32_t* o = P.list + at;
-        if (lane == 0) {
-          o[0] = nb;
-          o[1] = nr;
-        }
-        // L' ids: the task L is kept in m.T in list mode
-        uint32_t c = 0;
-        for (int w = 0; w < W; ++w) {
-          bool b = (ax[w] >> lane) & 1ull;
-          uint64_t bmk = ballot(b);
-          if (b) o[2 + c + lanes_below(bmk)] = m.T[w * 64 + lane];
-          c += __popcll(bmk);
-        }
-        for (uint32_t i = lane; i < nR0; i += kWave) o[2 + nb + i] = m.R[i];
-        c = nR0;
-        for (uint32_t base = 0; base < nC; base += kWave) {
+    if constexpr (!KL_R && !C_R) {
+      if (P.list) {
+        // R' = R0 ∪ {C[i] : a_i ⊇ L'} over the (ungrouped) root list
+        const R_t* root = (const R_t*)root_rec;
+        uint32_t nr = nR0;
+        for (uint32_t base = 0; base < root_len; base += kWave) {
           uint32_t i = base + lane;
           bool s = false;
-          if (i < nC) {
+          if (i < root_len) {
             s = true;
 #pragma unroll
             for (int w = 0; w < W; ++w) s = s && ((root[i].a[w] & ax[w]) == ax[w]);
           }
-          uint64_t sm = ballot(s);
-          if (s) o[2 + nb + c + lanes_below(sm)] = m.C[i];
-          c += __popcll(sm);
+          nr += __popcll(ballot(s));
+        }
+        uint64_t at = list_reserve(P, 2ull + nb + nr);
+        if (at != ~0ull) {
+          uint32_t* out = P.list + at;
+          if (lane == 0) {
+            out[0] = nb;
+            out[1] = nr;
+          }
+          // L' ids: the task L is kept in m.T in list mode
+          uint32_t c = 0;
+          for (int w = 0; w < W; ++w) {
+            bool b = (ax[w] >> lane) & 1ull;
+            uint64_t bmk = ballot(b);
+            if (b) out[2 + c + lanes_below(bmk)] = m.T[w * 64 + lane];
+            c += __popcll(bmk);
+          }
+          for (uint32_t i = lane; i < nR0; i += kWave) out[2 + nb + i] = m.R[i];
+          c = nR0;
+          for (uint32_t base = 0; base < root_len; base += kWave) {
+            uint32_t i = base + lane;
+            bool s = false;
+            if (i < root_len) {
+              s = true;
+#pragma unroll
+              for (int w = 0; w < W; ++w) s = s && ((root[i].a[w] & ax[w]) == ax[w]);
+            }
+            uint64_t sm = ballot(s);
+            if (s) out[2 + nb + c + lanes_below(sm)] = m.C[root[i].minidx];
+            c += __popcll(sm);
+          }
         }
       }
     }
-    if (nb > 1 && cfirst != kNone) {
+    if (nb > 1 && o.best_child != ~0ull) {
+      // commit the child list, then decide how to run it
+      const uint32_t sl = lds_top;
+      const uint64_t sh = hbm_top;
+      if (ctier == 0) lds_top = (uint32_t)(ctop + (uint64_t)o.clen * sizeof(R_t));
+      else hbm_top = ctop + (uint64_t)o.clen * sizeof(R_t);
+      bool here = child_node<W, KL_R, C_R>(P, m, gw, ctr, nL, nC, ta, crec, o.clen, o.best_child,
+                                           o.cbranch, nb, ax, sumR, lds_top, hbm_top,
+                                           fr + depth);
+      lds_top = sl;
+      hbm_top = sh;
+      if (!here) continue;
       // descend: the child list becomes a frame
-      if (lane == 0) m.fr[depth] = F;
+      if (lane == 0) fr[depth] = F;
       wave_mem_wait();
       ++depth;
-      if (ctier == 0) lds_top = (uint32_t)(ctop + (uint64_t)clen * sizeof(R_t));
-      else hbm_top = ctop + (uint64_t)clen * sizeof(R_t);
+      if (ctier == 0) lds_top = (uint32_t)(ctop + (uint64_t)o.clen * sizeof(R_t));
+      else hbm_top = ctop + (uint64_t)o.clen * sizeof(R_t);
       F.sumR = sumR;
       F.rec = crec_addr;
-      F.len = clen;
-      F.cur = cfirst;
+      F.len = o.clen;
+      F.cur = (uint32_t)o.best_child;
+      F.curidx = (uint32_t)(o.best_child >> 32);
       F.nbits = nb;
       F.tier = ctier;
       F.top = (uint32_t)ctop;
@@ -654,96 +1200,270 @@ __device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC,
   }
 }
 
-// Build the root list of a task (entry i: candidate i, branchable iff
-// i >= next_id, bitset of L ∩ N(C[i]) with Wt words; bitsets wider than
-// ceil(|L|/64) words are zero-padded) and run the stack machine.
+// Run the items of one entry (`word`: lane l holds word l).
+template <int W>
+__device__ void run_items_w(const Params& P, const WaveMem& m, CtxHdr* h, uint32_t nL,
+                            uint32_t nC, uint32_t len, uint32_t nbits, uint64_t sumR0,
+                            uint32_t word, uint32_t cnt, Counters& ctr, uint64_t gw) {
+  typedef Rec<W> R_t;
+  R_t* recs;
+  TaskArrays ta;
+  ctx_arrays<W>(h, nL, nC, len, recs, ta);
+  for (uint32_t it = 0; it < cnt; ++it) {
+    const uint32_t xpos = read_lane(word, 2 + 2 * it), xidx = read_lane(word, 3 + 2 * it);
+    uint64_t t0 = P.profile ? cycles() : 0;
+    {
+      uint32_t xc = ld_dev(ta.C + xidx);
+      uint64_t dx = (uint64_t)(P.g.v_off[xc + 1] - P.g.v_off[xc]);
+      ctr.balg += 4ull * (dx + nbits + nC);
+    }
+    uint64_t ax[W];
+    uint32_t nb = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      ax[w] = uni64(ld_dev(&recs[xpos].a[w]));
+      nb += __popcll(ax[w]);
+    }
+    uint32_t lds_top = 0;
+    uint64_t hbm_top = 0;
+    uint32_t ctier = 1;
+    uint64_t ctop = 0;
+    uint64_t crec_addr = arena_take(m, lds_top, hbm_top, (uint64_t)len * sizeof(R_t), ctier, ctop);
+    if (crec_addr == 0) {
+      ctr.overflow++;
+      ctr.ovf_arena++;
+      continue;
+    }
+    uint64_t tp = P.profile ? cycles() : 0;
+    PassOut o = push_pass<W, true>(recs, len, ax, xidx, (R_t*)crec_addr, false);
+    if (P.profile) ctr.c_pass += cycles() - tp;
+    ctr.nodes++;
+    ctr.entries += len;
+    if (W == 1) {
+      ctr.ent_w1 += len;
+      ctr.nodes_w1++;
+    }
+    if (o.nonmax) {
+      ctr.nonmax++;
+      phase(ctr, P.profile, 5, t0);
+      continue;
+    }
+    const uint64_t sumL = key_sum_of_bits<W, true>(ax, ta);
+    const uint64_t sumR = sumR0 + o.absorbed;
+    add_biclique(ctr, biclique_hash(sumL, sumR, P.g.swapped));
+    if (nb > 1 && o.best_child != ~0ull) {
+      if (ctier == 0) lds_top = (uint32_t)(ctop + (uint64_t)o.clen * sizeof(R_t));
+      else hbm_top = ctop + (uint64_t)o.clen * sizeof(R_t);
+      bool here = child_node<W, true, true>(P, m, gw, ctr, nL, nC, ta, (R_t*)crec_addr, o.clen,
+                                            o.best_child, o.cbranch, nb, ax, sumR, lds_top,
+                                            hbm_top, m.fr);
+      if (here) {
+        if (ctier == 0) lds_top = (uint32_t)(ctop + (uint64_t)o.clen * sizeof(R_t));
+        else hbm_top = ctop + (uint64_t)o.clen * sizeof(R_t);
+        Frame F;
+        F.sumR = sumR;
+        F.rec = crec_addr;
+        F.len = o.clen;
+        F.cur = (uint32_t)o.best_child;
+        F.curidx = (uint32_t)(o.best_child >> 32);
+        F.nbits = nb;
+        F.tier = ctier;
+        F.top = (uint32_t)ctop;
+        dfs<W, true, true>(P, m, nL, nC, F, lds_top, hbm_top, ctr, ta, 0, 0, 0, gw, m.fr);
+      }
+    }
+    phase(ctr, P.profile, 5, t0);
+  }
+  if (lane_id() == 0)
+    __hip_atomic_fetch_add(&h->refs, (unsigned long long)(-(long long)cnt), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int WMAX>
+__device__ void run_items(const Params& P, const WaveMem& m, uint32_t word, Counters& ctr,
+                          uint64_t gw) {
+  const uint32_t cnt = read_lane(word, 0) & ~kItemKind;
+  const uint32_t owner = read_lane(word, 1);
+  CtxHdr* h = ctx_hdr(P, owner);
+  const uint32_t nL = uni(ld_dev(&h->nL)), nC = uni(ld_dev(&h->nC)), len = uni(ld_dev(&h->len)),
+                 W = uni(ld_dev(&h->W));
+  const uint32_t nbits = uni(ld_dev(&h->nbits));
+  const uint64_t sumR0 = uni64(ld_dev(&h->sumR0));
+  if (W == 1) run_items_w<1>(P, m, h, nL, nC, len, nbits, sumR0, word, cnt, ctr, gw);
+  else if (W == 2) run_items_w<2>(P, m, h, nL, nC, len, nbits, sumR0, word, cnt, ctr, gw);
+  else if (W == 4) run_items_w<4>(P, m, h, nL, nC, len, nbits, sumR0, word, cnt, ctr, gw);
+  else if constexpr (WMAX >= 16) {
+    if (W == 8) run_items_w<8>(P, m, h, nL, nC, len, nbits, sumR0, word, cnt, ctr, gw);
+    else run_items_w<16>(P, m, h, nL, nC, len, nbits, sumR0, word, cnt, ctr, gw);
+  }
+}
+
+// Build the root list of a task (entry i starts as candidate i alone:
+// minidx = i, rep = i if i >= next_id else kNone, bitset of L ∩ N(C[i]) with
+// Wt words, zero-padded beyond |L|; with one-word bitsets the candidates are
+// then grouped by bitset through a per-wave hash table), then either publish
+// it as a shared context (share) or run the stack machine here.
 template <int Wt>
 __device__ void enumerate_task(const Params& P, const WaveMem& m, const uint32_t* L, uint32_t nL,
                                uint32_t nC, uint32_t next_id, uint64_t sumR, uint32_t nR,
-                               Counters& ctr) {
+                               Counters& ctr, uint64_t& t0, bool share, uint64_t gw) {
+  typedef Rec<Wt> R_t;
   const DevGraph& g = P.g;
   const uint32_t lane = lane_id();
   int32_t* cmap = m.cmap;
-  const uint64_t rs = sizeof(Rec<Wt>);
   uint32_t lds_top = 0;
   uint64_t hbm_top = 0;
-  uint32_t rtier = 1;
-  uint64_t rtop = 0;
-  uint64_t root = arena_take(m, lds_top, hbm_top, (uint64_t)nC * rs, rtier, rtop);
-  if (root == 0) {
+  // candidate bitsets, one record per candidate, built in the HBM arena
+  if ((uint64_t)nC * sizeof(R_t) > m.arena_bytes) {
     ctr.overflow++;
+    ctr.ovf_arena++;
     for (uint32_t i = lane; i < nC; i += kWave) st_wg(cmap + m.C[i], 0);
     wave_mem_wait();
     return;
   }
-  if (rtier == 0) lds_top = (uint32_t)(rtop + (uint64_t)nC * rs);
-  else hbm_top = rtop + (uint64_t)nC * rs;
-  uint64_t* rw = (uint64_t*)root;
-  const uint64_t stride = Wt + 1;
-  for (uint64_t i = lane; i < (uint64_t)nC * stride; i += kWave) {
-    uint64_t e = i / stride, w = i % stride;
-    rw[i] = w == 0 ? (uint64_t)((uint32_t)e | (e >= next_id ? kBranch : 0u)) : 0ull;
+  const uint64_t braw = (uint64_t)m.arena;
+  hbm_top = (uint64_t)nC * sizeof(R_t);
+  R_t* brec = (R_t*)braw;
+  for (uint32_t i = lane; i < nC; i += kWave) {
+    brec[i].minidx = i;
+    brec[i].rep = i >= next_id ? i : kNone;
+    brec[i].ksum = m.kC[i];
+#pragma unroll
+    for (int w = 0; w < Wt; ++w) brec[i].a[w] = 0ull;
   }
   wave_mem_wait();
   for_each_edge(L, nL, g.u_off, g.u_adj, m.s_off, m.s_start, [&](uint32_t j, uint32_t w) {
     int32_t p = ld_wg(cmap + w);
-    if (p < 0) {
-      uint64_t* word = rw + (uint64_t)(-p - 1) * stride + 1 + (j >> 6);
-      or_wg(word, 1ull << (j & 63));
-    }
+    if (p < 0) or_wg(&brec[-p - 1].a[j >> 6], 1ull << (j & 63));
   });
   wave_mem_wait();
   for (uint32_t i = lane; i < nC; i += kWave) st_wg(cmap + m.C[i], 0);
   wave_mem_wait();
-  dfs<Wt>(P, m, nL, nC, next_id, sumR, root, rtier, rtop, lds_top, hbm_top, ctr, nR);
+  // group twins (not in list mode: its R' lists need one entry per candidate)
+  uint32_t len = nC;
+  const uint64_t list_addr = braw;
+  if (P.group && !P.list && nC > 1) {
+    uint64_t bc = 0;
+    uint32_t cb = 0;
+    len = merge_twins<Wt>(m, brec, nC, lds_top, hbm_top, bc, cb);
+  }
+  wave_mem_wait();
+  TaskArrays lta{m.kL, m.C};
+  if (share && publish_list<Wt, false, false>(P, gw, (const R_t*)list_addr, len, nL, nC, nL, sumR,
+                                              lta)) {
+    ctr.splits++;
+    phase(ctr, P.profile, 6, t0);
+    return;
+  }
+  // final root list: into LDS when it fits, else at the HBM arena's base
+  uint64_t root = list_addr;
+  uint32_t rtier = 1;
+  if ((uint64_t)len * sizeof(R_t) <= m.lds_bytes) {
+    R_t* dst = (R_t*)m.lds;
+    const R_t* src = (const R_t*)list_addr;
+    for (uint32_t i = lane; i < len; i += kWave) dst[i] = src[i];
+    root = (uint64_t)m.lds;
+    rtier = 0;
+    lds_top = (uint32_t)((uint64_t)len * sizeof(R_t));
+    hbm_top = 0;
+  } else {
+    hbm_top = (uint64_t)len * sizeof(R_t);
+  }
+  wave_mem_wait();
+  // first push: the smallest rep
+  uint64_t best = ~0ull;
+  const R_t* rl = (const R_t*)root;
+  for (uint32_t i = lane; i < len; i += kWave) {
+    uint32_t rp = rl[i].rep;
+    if (rp != kNone) {
+      uint64_t c = ((uint64_t)rp << 32) | i;
+      best = c < best ? c : best;
+    }
+  }
+  best = uni64(wave_min64(best));
+  phase(ctr, P.profile, 4, t0);
+  if (best != ~0ull) {
+    Frame F;
+    F.sumR = sumR;
+    F.rec = root;
+    F.len = len;
+    F.cur = (uint32_t)best;
+    F.curidx = (uint32_t)(best >> 32);
+    F.nbits = nL;
+    F.tier = rtier;
+    F.top = 0;
+    TaskArrays ta{m.kL, m.C};
+    dfs<Wt, false, false>(P, m, nL, nC, F, lds_top, hbm_top, ctr, ta, root, len, nR, gw, m.fr);
+  }
+  phase(ctr, P.profile, 5, t0);
 }
 
 // ---------------------------------------------------------------------------
 // one task
 
+// Common neighbourhood of a prefix S[0..kp) (the L of a split parent), shared
+// by all children of one queue entry.  kp = 1 points into the CSR row.
+struct Prefix {
+  const uint32_t* L;
+  uint32_t n;
+  uint64_t sdeg;  // sum of the prefix vertices' degrees
+};
+
+__device__ Prefix prefix_common(const DevGraph& g, const WaveMem& m, const Task& t, int kp) {
+  const uint32_t lane = lane_id();
+  Prefix p;
+  uint32_t v0 = t.v[0];
+  p.sdeg = (uint64_t)(g.v_off[v0 + 1] - g.v_off[v0]);
+  p.n = (uint32_t)(g.v_off[v0 + 1] - g.v_off[v0]);
+  const uint32_t* cur = g.v_adj + g.v_off[v0];
+  for (int i = 1; i < kp; ++i) {
+    uint32_t vi = t.v[i];
+    uint32_t di = (uint32_t)(g.v_off[vi + 1] - g.v_off[vi]);
+    p.sdeg += di;
+    uint32_t* dst = (i & 1) ? m.T : m.Lp;
+    p.n = wave_intersect(cur, p.n, g.v_adj + g.v_off[vi], di, dst);
+    cur = dst;
+  }
+  if (kp > 1 && cur != m.Lp) {
+    for (uint32_t i = lane; i < p.n; i += kWave) m.Lp[i] = cur[i];
+    wave_sync();
+    cur = m.Lp;
+  }
+  p.L = cur;
+  return p;
+}
+
+// One task S = t.v[0..k).  For k > 1, `pre` is the common neighbourhood of
+// S[0..k-1) (prefix_common).
 template <int WMAX>
 __device__ void run_task(const Params& P, const WaveMem& m, const Task& t, int k,
-                         Counters& ctr) {
+                         const Prefix& pre, Counters& ctr, uint64_t gw) {
   const DevGraph& g = P.g;
   const uint32_t lane = lane_id();
   ctr.tasks++;
   ctr.nodes++;
+  uint64_t t0 = P.profile ? cycles() : 0;
   // L = ∩ N(S) (get_common_neighbors, src/IterFinderGpu.cu:680-711)
   const uint32_t* L;
-  uint32_t nL, np = 0;
+  uint32_t nL;
+  const uint32_t np = k > 1 ? pre.n : 0;
   const uint32_t vk = t.v[k - 1];
+  const uint32_t dk = (uint32_t)(g.v_off[vk + 1] - g.v_off[vk]);
+  uint64_t sdeg = dk;  // sum of the task vertices' degrees
   if (k == 1) {
     L = g.v_adj + g.v_off[vk];
-    nL = (uint32_t)(g.v_off[vk + 1] - g.v_off[vk]);
+    nL = dk;
   } else {
-    // prefix: S[0..k-1) into Lp, then ∩ N(vk) into L
-    uint32_t v0 = t.v[0];
-    np = (uint32_t)(g.v_off[v0 + 1] - g.v_off[v0]);
-    const uint32_t* cur = g.v_adj + g.v_off[v0];
-    uint32_t* bufA = m.Lp;
-    uint32_t* bufB = m.T;
-    for (int i = 1; i < k - 1; ++i) {
-      uint32_t vi = t.v[i];
-      uint32_t* dst = (i & 1) ? bufB : bufA;
-      np = wave_intersect(cur, np, g.v_adj + g.v_off[vi], (uint32_t)(g.v_off[vi + 1] - g.v_off[vi]),
-                          dst);
-      cur = dst;
-    }
-    if (cur != m.Lp) {
-      for (uint32_t i = lane; i < np; i += kWave) m.Lp[i] = cur[i];
-      wave_sync();
-    }
-    nL = wave_intersect(m.Lp, np, g.v_adj + g.v_off[vk], (uint32_t)(g.v_off[vk + 1] - g.v_off[vk]),
-                        m.L);
+    sdeg += pre.sdeg;
+    nL = wave_intersect(pre.L, pre.n, g.v_adj + g.v_off[vk], dk, m.L);
     L = m.L;
   }
-  uint64_t sdeg = 0;  // sum of the prefix vertices' degrees
-  for (int i = 0; i < k; ++i) sdeg += (uint64_t)(g.v_off[t.v[i] + 1] - g.v_off[t.v[i]]);
   if (nL == 0) {  // isolated vertex: no biclique (undefined in the reference, Q5)
     ctr.balg += 4ull * sdeg;
     ctr.nonmax++;
     return;
   }
+  phase(ctr, P.profile, 1, t0);
   // 2-hop counts
   int32_t* cmap = m.cmap;
   uint64_t* bm0 = m.bm0;
@@ -758,7 +1478,9 @@ __device__ void run_task(const Params& P, const WaveMem& m, const Task& t, int k
     }
   });
   ctr.balg += 4ull * (sdeg + root_edges);
+  ctr.edges += root_edges;
   wave_mem_wait();
+  phase(ctr, P.profile, 2, t0);
   // sorted extraction of the 2-hop set with its counts; clears the bitmaps
   uint32_t nU2 = 0;
   for (uint32_t b1 = 0; b1 < P.w.nW1; b1 += kWave) {
@@ -808,16 +1530,24 @@ __device__ void run_task(const Params& P, const WaveMem& m, const Task& t, int k
   if (k == 1) {
     canonical = nR > 0 && m.R[0] == vk;
   } else {
+    // every w in R below vk outside the prefix must contain the prefix's L
     canonical = true;
-    for (uint32_t i = 0; i < nR && canonical; ++i) {
-      uint32_t w = uni(m.R[i]);
-      if (w >= vk) break;
+    for (uint32_t base = 0; base < nR && canonical; base += kWave) {
+      uint32_t i = base + lane;
+      uint32_t w = i < nR ? m.R[i] : kNone;
+      bool below = w < vk;
       bool in_prefix = false;
       for (int j = 0; j < k - 1; ++j) in_prefix |= (t.v[j] == w);
-      if (in_prefix) continue;
-      uint32_t c = wave_intersect_count(m.Lp, np, g.v_adj + g.v_off[w],
-                                        (uint32_t)(g.v_off[w + 1] - g.v_off[w]));
-      canonical = uni(c) == np;
+      uint64_t cand = ballot(below && !in_prefix);
+      while (cand && canonical) {
+        uint32_t wc = read_lane(w, first_lane(cand));
+        cand &= cand - 1;
+        uint32_t c = wave_intersect_count(pre.L, np, g.v_adj + g.v_off[wc],
+                                          (uint32_t)(g.v_off[wc + 1] - g.v_off[wc]));
+        canonical = uni(c) == np;
+      }
+      if (!ballot(below)) break;
+      if (ballot(!below)) break;
     }
   }
   auto clear_cmap = [&]() {
@@ -853,25 +1583,38 @@ __device__ void run_task(const Params& P, const WaveMem& m, const Task& t, int k
   };
   uint64_t sumL = 0, sumR = 0;
   uint32_t rest = nC - next_id;
+  phase(ctr, P.profile, 3, t0);
   if (nL <= 1 || rest == 0) {
     count_root(sumL, sumR);
     clear_cmap();
     return;
   }
   const uint32_t W = (nL + 63) / 64;
-  // split (src/IterFinderGpu.cu:835-858): children S + [C[i]], i >= next_id
+  // split (src/IterFinderGpu.cu:835-858)
   bool rule = P.split && ((rest > (uint32_t)P.bound_height || nL > (uint32_t)P.bound_height) &&
                           (uint64_t)rest * nL > (uint64_t)P.bound_size);
   bool must = W > (uint32_t)kMaxW;  // |L| beyond any bitset: split regardless
-  if ((rule && k < P.max_prefix) || (must && k < kMaxPrefix)) {
+  // large task: share its root list with other waves when this wave's context
+  // area is free and large enough
+  bool share = rule && P.share && !P.list && W <= (uint32_t)WMAX &&
+               ld_dev(&ctx_hdr(P, gw)->refs) == 0ull;
+  if (share) {
+    const uint32_t Wt = W <= 1 ? 1 : (W <= 2 ? 2 : (W <= 4 ? 4 : (W <= 8 ? 8 : 16)));
+    const uint64_t bytes = sizeof(CtxHdr) + (uint64_t)nC * (16 + 8 * Wt + 12) + (uint64_t)nL * 8;
+    share = bytes <= P.w.ctx_bytes;
+  }
+  if (!share && ((rule && k < P.max_prefix) || (must && k < kMaxPrefix))) {
+    // children S + [C[i]], i >= next_id, each recomputed by its wave
     const unsigned long long ne = (rest + kChunk - 1) / kChunk;
     if (lane == 0)
       __hip_atomic_fetch_add(P.s.work_left, ne, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t pushed = 0;
-    if (queue_push_children(P.s, t, k, m.C + next_id, rest, pushed)) {
+    if (queue_push_children(P.s, t, k, m.C + next_id, rest, pushed,
+                            (uint32_t)(gw & (kShards - 1)))) {
       ctr.splits++;
       count_root(sumL, sumR);
       clear_cmap();
+      phase(ctr, P.profile, 6, t0);
       return;
     }
     if (lane == 0)
@@ -880,6 +1623,7 @@ __device__ void run_task(const Params& P, const WaveMem& m, const Task& t, int k
   }
   if (must) {
     ctr.overflow++;
+    ctr.ovf_wide++;
     clear_cmap();
     return;
   }
@@ -898,6 +1642,7 @@ __device__ void run_task(const Params& P, const WaveMem& m, const Task& t, int k
     at = uni64(at);
     if (at == ~0ull) {
       ctr.overflow++;
+      ctr.ovf_defer++;
       return;
     }
     if (lane < (uint32_t)kMaxPrefix) st_dev(&P.s.deferred[at].v[lane], t.v[lane]);
@@ -915,14 +1660,14 @@ __device__ void run_task(const Params& P, const WaveMem& m, const Task& t, int k
     if (P.list) m.T[j] = L[j];
   }
   if (W <= 1) {
-    enumerate_task<1>(P, m, L, nL, nC, next_id, sumR, nR, ctr);
+    enumerate_task<1>(P, m, L, nL, nC, next_id, sumR, nR, ctr, t0, share, gw);
   } else if (W <= 2) {
-    enumerate_task<2>(P, m, L, nL, nC, next_id, sumR, nR, ctr);
+    enumerate_task<2>(P, m, L, nL, nC, next_id, sumR, nR, ctr, t0, share, gw);
   } else if (W <= 4) {
-    enumerate_task<4>(P, m, L, nL, nC, next_id, sumR, nR, ctr);
+    enumerate_task<4>(P, m, L, nL, nC, next_id, sumR, nR, ctr, t0, share, gw);
   } else if constexpr (WMAX >= 16) {
-    if (W <= 8) enumerate_task<8>(P, m, L, nL, nC, next_id, sumR, nR, ctr);
-    else enumerate_task<16>(P, m, L, nL, nC, next_id, sumR, nR, ctr);
+    if (W <= 8) enumerate_task<8>(P, m, L, nL, nC, next_id, sumR, nR, ctr, t0, share, gw);
+    else enumerate_task<16>(P, m, L, nL, nC, next_id, sumR, nR, ctr, t0, share, gw);
   }
 }
 

@@ -48,6 +48,16 @@ class IterFinderGpu6:
         check(lib().gmbe_finder_list(self._h, buf.ctypes.data, n, C.byref(used)))
         return records_to_set(buf[:n])
 
+    STAT_NAMES = ["cyc_wait", "cyc_prefix", "cyc_expand", "cyc_extract", "cyc_build", "cyc_dfs",
+                  "cyc_split", "edges", "entries", "ovf_arena", "ovf_wide", "ovf_defer",
+                  "entries_w1", "nodes_w1", "cyc_tail", "sub_pass", "sub_merge",
+                  "sub_compress", "sub_publish", "publishes", "merges"]
+
+    def stats(self) -> dict:
+        out = (C.c_uint64 * 32)()
+        n = lib().gmbe_finder_stats(self._h, out, 32)
+        return {k: int(out[i]) for i, k in enumerate(self.STAT_NAMES[:n])}
+
     def print_result(self):
         r = self.result
         print(f"non_maximal: {r.non_maximal}")

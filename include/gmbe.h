@@ -188,6 +188,21 @@ int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res);
  * GMBE_ERR_CAPACITY if cap is too small or the device buffer overflowed. */
 int gmbe_finder_list(gmbe_finder* f, uint32_t* buf, size_t cap, size_t* used);
 
+/* Profile of the last execute (no reference counterpart; the reference's
+ * -p prints per-SM exit times, src/IterFinderGpu.cu:1038-1041).  Copies up to n
+ * counters: [0] waiting for work, [1] prefix intersections, [2] 2-hop edge
+ * pass, [3] extraction + canonicity, [4] candidate-list build, [5] stack
+ * machine, [6] split pushes (cycles summed over waves, filled when the knob
+ * "profile" is 1); [7] edges of the 2-hop passes, [8] list entries scanned by
+ * pushes; [9..11] overflows by cause (list arena, |L| too wide, deferred
+ * buffer); [12] list entries and [13] pushes in one-word-bitset lists;
+ * [14] cycles of the final wait (a wave with nothing left to do);
+ * [15..18] cycles inside the stack machine spent in push passes, twin
+ * merges, bitset re-indexing and context publishing; [19] contexts
+ * published, [20] twin merges.
+ * Returns the number of counters available. */
+int gmbe_finder_stats(gmbe_finder* f, uint64_t* out, size_t n);
+
 /* Replaces the finder destructor chain (src/IterFinderGpu.cu:1087-1093). */
 void gmbe_finder_destroy(gmbe_finder* f);
 

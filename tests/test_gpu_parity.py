@@ -161,3 +161,30 @@ def test_full_size_config2_root_sample(gpu, stride, phase):
     res = oracle.gmbe(g.oracle_graph(), threads=8, stride=stride, phase=phase)
     r = gpu_digest(g, root_stride=stride, root_phase=phase)
     assert r.digest() == res.digest()
+
+
+@pytest.mark.parametrize("knobs", [
+    dict(share=0), dict(share=1), dict(group=0), dict(group=1, merge_min=2),
+    dict(share=1, share_work=1, share_queue=1 << 30),       # share every node it can
+    dict(share=1, bound_height=0, bound_size=0, merge_min=2),
+    dict(prune=0),
+])
+def test_scheduler_knobs_keep_the_result(gpu, knobs):
+    """Sharing, grouping, twin merging and pruning change the traversal only."""
+    g = workloads.run_graph(2, 0.01)
+    base = oracle.gmbe(g.oracle_graph(), threads=8)
+    r = gpu_digest(g, **knobs)
+    assert r.digest() == base.digest()
+
+
+def test_unsplit_matches_reference_traversal(gpu):
+    """Without splitting, the pushes, non-maximal verdicts and algorithmic
+    bytes are those of the reference's traversal (the oracle restates it)."""
+    g = workloads.run_graph(1)
+    o = oracle.gmbe(g.oracle_graph())
+    for group in (0, 1):
+        r = gpu_digest(g, sel=0, group=group)
+        assert r.digest() == o.digest()
+        assert r.nodes == o.nodes
+        assert r.non_maximal == o.non_maximal
+        assert r.alg_bytes == o.b_alg
