@@ -1,16 +1,25 @@
 """Benchmark: maximal biclique enumeration on one MI355X.
 
-A step is one full enumeration (IterFinderGpu6::Execute) of the power-law
-bipartite graph (Chung-Lu gamma=2.1, |U|=100,000, |V|=200,000, 1.5M distinct
-edges, degree cap 20,000, seed 2; run as -t 1 -o 1 -f, bound_height=20,
-bound_size=1500) with the graph already resident in HBM.  The line reports
+Workload (a step): the power-law bipartite graph at full size (Chung-Lu
+gamma=2.1 on both sides, |U|=100,000, |V|=200,000, 1.5M distinct edges, degree
+cap 20,000, seed 2; loaded as -t 1 -o 1 -f; bound_height=20, bound_size=1500),
+enumerated through IterFinderGpu6::Execute over the top-level vertices of
+processing rank >= 4096, i.e. the 95,904 lightest of the 100,000 task roots.
+The 4,096 heaviest roots are each a separate multi-minute enumeration (the
+graph's hub core yields more than 6.8e8 maximal bicliques there); a full run
+over them does not fit a benchmark step and is reported beside this one,
+measured in rank ranges with the finder's deadline knob.
+
+The graph is resident in HBM before the timed region.  The JSON line reports
 maximal bicliques per second over the timed steps, the roofline of the
-enumeration kernel (algorithmic bytes per launch / kernel time, against the
-8 TB/s HBM peak) and the CPU baseline: the restatement of the reference's CPU
-finder (MbeaFinder with MbeaAdvFinder's 2-hop top level) timed on this host's
-cores over a bounded sample of top-level vertices of the same graph.
+enumeration kernel (algorithmic bytes per launch / kernel time from HIP events
+on the finder's stream, against the 8 TB/s HBM peak) and the CPU baseline:
+the restatement of the reference's CPU finder (MbeaFinder with MbeaAdvFinder's
+2-hop top level) timed on this host's cores over a bounded sample of the same
+top-level vertices.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py --copy-calibration   # plain 1 GiB device copy, for HBM counters
 """
 from __future__ import annotations
 
@@ -26,50 +35,89 @@ sys.path.insert(0, ROOT)
 METRIC = ("MBE wall time + maximal bicliques/s on 1 MI355X; p50/p99 latency of served "
           "queries")
 HBM_PEAK = 8.0e12  # bytes/s, MI355X
+ROOT_BEGIN = 4096  # first processing rank of the benchmark step
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "mbe_persistent_config2_hbm_traffic.json")
 
 
-def cpu_baseline(graph, seconds: float):
-    """The oracle's MbeaFinder restatement on a root sample of the same graph."""
+def workload_name(config: int, scale: float, root_begin: int) -> str:
+    base = ("Chung-Lu gamma=2.1 |U|=100,000 |V|=200,000 1.5M edges cap 20,000 seed 2"
+            if config == 2 else f"config {config} graph of gmbe_amd/workloads.py")
+    if scale != 1.0:
+        base += f", sizes x{scale:g}"
+    return (f"MBE {base}; -t 1 -o 1 -f; bound_height=20 bound_size=1500; "
+            f"top-level vertices of processing rank >= {root_begin}")
+
+
+def cpu_baseline(graph, root_begin: int, seconds: float):
+    """The oracle's MbeaFinder restatement on a sample of the step's roots."""
     import oracle
     threads = min(16, os.cpu_count() or 1)
     stride = 97
     og = graph.oracle_graph()
-    # grow the sample until it costs a few seconds, bounded by `seconds`
-    best = None
-    for phase in range(stride):
-        r = oracle.mbea(og, threads=threads, stride=stride, phase=phase,
-                        time_limit=max(1.0, seconds))
-        best = r if best is None else best
-        if best is not r:
-            best.maximal += r.maximal
-            best.seconds += r.seconds
-            best.tasks += r.tasks
-        if best.seconds >= seconds:
+    maximal, secs, tasks, phases = 0, 0.0, 0, 0
+    for j in range(stride):
+        r = oracle.mbea(og, threads=threads, stride=stride, phase=root_begin + j,
+                        time_limit=max(1.0, seconds - secs))
+        maximal += r.maximal
+        secs += r.seconds
+        tasks += r.tasks
+        phases += 1
+        if secs >= seconds:
             break
-    rate = best.maximal / best.seconds if best.seconds > 0 else 0.0
     return {
-        "value": rate, "unit": "bicliques/s", "cores": threads, "kind": "port",
-        "sample": (f"MbeaFinder restatement (oracle/oracle.c orc_mbea) over {best.tasks} "
-                   f"top-level vertices (every {stride}th in processing order, time-limited "
-                   f"to ~{seconds:.0f} s): {best.maximal} bicliques in {best.seconds:.1f} s"),
+        "value": maximal / secs if secs > 0 else 0.0, "unit": "bicliques/s", "cores": threads,
+        "kind": "port",
+        "sample": (f"MbeaFinder restatement (oracle/oracle.c orc_mbea, {threads} threads) over "
+                   f"{tasks} top-level vertices of the same step (ranks {root_begin}+j+97t, "
+                   f"j < {phases}): {maximal} maximal bicliques in {secs:.1f} s"),
     }
+
+
+def copy_calibration():
+    """A plain device copy of 1 GiB (read 1 GiB, write 1 GiB), three times,
+    so that the HBM counters of a profiler can be scaled to bytes."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
+
+
+def measured_traffic(workload: str):
+    """HBM bytes per launch of the enumeration kernel from the committed
+    counter summary, when it was taken on this workload."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload:
+        return None
+    return t.get("hbm_bytes_per_launch")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--root-begin", type=int, default=ROOT_BEGIN)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--lds-kb", type=int, default=8)
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is designed but not built", file=sys.stderr)
+              "sharding over several GPUs is designed but not built", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     from gmbe_amd import IterFinderGpu6, workloads
 
@@ -78,7 +126,8 @@ def main():
     st = graph.stats()
     print(f"# graph: |U|={st['U_size']} |V|={st['V_size']} |E|={st['edge_size']} "
           f"(built in {time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
-    f = IterFinderGpu6(graph, workloads.BOUND_HEIGHT, workloads.BOUND_SIZE, lds_kb=args.lds_kb)
+    f = IterFinderGpu6(graph, workloads.BOUND_HEIGHT, workloads.BOUND_SIZE,
+                       root_begin=args.root_begin)
     for i in range(args.warmup):
         r = f.execute()
         print(f"# warmup {i}: {r.maximal} bicliques in {r.exe_time_s:.3f} s", file=sys.stderr,
@@ -91,6 +140,7 @@ def main():
         print(f"# step {i}: {r.maximal} bicliques, exe {r.exe_time_s:.3f} s, kernel "
               f"{r.kernel_time_s:.3f} s", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t_start
+    f.close()
     digests = {r.digest() for r in results}
     if len(digests) != 1:
         raise SystemExit(f"bench: steps disagree: {digests}")
@@ -98,6 +148,7 @@ def main():
     total = sum(r.maximal for r in results)
     kernel_avg = sum(r.kernel_time_s for r in results) / len(results)
     achieved = r0.alg_bytes / kernel_avg if kernel_avg > 0 else 0.0
+    workload = workload_name(args.config, args.scale, args.root_begin)
     line = {
         "metric": METRIC,
         "value": total / elapsed,
@@ -109,14 +160,12 @@ def main():
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "uint64",
-        "data": "synthetic",
+        "dtype": "u64",
+        "data": "synthetic (seeded generator, built in-process)",
         "config": {
-            "workload": (f"config{args.config} x{args.scale:g}: " +
-                         ("Chung-Lu gamma=2.1 |U|=100,000 |V|=200,000 1.5M edges cap 20,000 "
-                          "seed 2" if args.config == 2 else "see gmbe_amd/workloads.py") +
-                         "; -t 1 -o 1 -f; bound_height=20 bound_size=1500"),
+            "workload": workload,
             "U_size": st["U_size"], "V_size": st["V_size"], "edge_size": st["edge_size"],
+            "task_roots": st["V_size"] - min(args.root_begin, st["V_size"]),
             "maximal_bicliques": int(r0.maximal),
             "hash_sum": f"{r0.hash_sum:#018x}", "hash_xor": f"{r0.hash_xor:#018x}",
             "non_maximal": int(r0.non_maximal), "nodes": int(r0.nodes),
@@ -125,14 +174,13 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": None,
+            "frac": achieved / HBM_PEAK, "traffic": measured_traffic(workload),
             "alg_bytes_per_launch": int(r0.alg_bytes),
         },
     }
-    f.close()
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(graph, args.cpu_seconds)
-    print(json.dumps(line))
+        line["cpu_baseline"] = cpu_baseline(graph, args.root_begin, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

@@ -12,6 +12,7 @@ namespace gmbe {
 struct Engine {
   int device = -1;
   int cu_count = 0;
+  int wall_khz = 100000;  // rate of the device's constant wall clock
   int ensure_device(int dev);
 };
 
@@ -28,6 +29,6 @@ struct DeviceGraph {
 };
 
 mbe::Scratch scratch_layout(uint32_t nV, uint32_t dmaxV, uint64_t arena_bytes,
-                            uint64_t ctx_bytes);
+                            uint64_t ctx_bytes, uint32_t ctx_slots);
 
 }  // namespace gmbe

@@ -19,8 +19,12 @@ using namespace gmbe;
 namespace mbe {
 
 // Persistent grid: every wave loops over tasks until none are left.
-template <int WAVES, int LDS_PER_WAVE, int WMAX>
-__global__ __launch_bounds__(WAVES * 64) void mbe_persistent(Params P) {
+// OCC: waves per SIMD the register allocation must leave room for (also
+// imposed on the device functions the kernel calls).
+template <int WAVES, int LDS_PER_WAVE, int WMAX, int OCC>
+__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
+void mbe_persistent(Params P) {
+  if (P.budget != 0) P.deadline = wall_clock64() + P.budget;
   __shared__ __attribute__((aligned(16))) char lds[WAVES * (LDS_PER_WAVE > 0 ? LDS_PER_WAVE : 16)];
   __shared__ uint32_t s_off[WAVES][kWave + 1];
   __shared__ int64_t s_start[WAVES][kWave];
@@ -110,7 +114,19 @@ __global__ __launch_bounds__(WAVES * 64) void mbe_persistent(Params P) {
                            __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_fetch_add(a + A_OVF_DEFER, (unsigned long long)c.ovf_defer, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(a + A_ABANDONED, (unsigned long long)c.abandoned, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(a + A_LATE, (unsigned long long)c.late, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
   }
+}
+
+// Mark every shared-context slot free (a run cut short by its deadline can
+// leave items of a context unrun).
+__global__ void reset_ctx(Params P, unsigned long long n) {
+  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (unsigned long long)gridDim.x * blockDim.x)
+    ctx_hdr(P, i)->refs = 0ull;
 }
 
 // Reset the queue's slot sequence numbers (slot i of every shard starts free
@@ -149,16 +165,19 @@ struct KernelChoice {
   int lds_per_wave;
 };
 
-static KernelChoice pick_kernel(int lds_kb, bool wide = false) {
+static KernelChoice pick_kernel(int lds_kb, int occ, bool wide = false) {
   using namespace mbe;
-  if (wide) return {mbe_persistent<4, 8192, kMaxW>, 4, 8192};
+  // every kernel shares the stack-machine functions, whose registers are
+  // allocated for the laxest caller: all of them ask for two waves per SIMD
+  (void)occ;
+  if (wide) return {mbe_persistent<4, 8192, kMaxW, 2>, 4, 8192};
   switch (lds_kb) {
-    case 0: return {mbe_persistent<4, 0, kMainW>, 4, 0};
-    case 4: return {mbe_persistent<4, 4096, kMainW>, 4, 4096};
-    case 12: return {mbe_persistent<4, 12288, kMainW>, 4, 12288};
-    case 16: return {mbe_persistent<4, 16384, kMainW>, 4, 16384};
-    case 24: return {mbe_persistent<4, 24576, kMainW>, 4, 24576};
-    default: return {mbe_persistent<4, 8192, kMainW>, 4, 8192};
+    case 0: return {mbe_persistent<4, 0, kMainW, 2>, 4, 0};
+    case 4: return {mbe_persistent<4, 4096, kMainW, 2>, 4, 4096};
+    case 12: return {mbe_persistent<4, 12288, kMainW, 2>, 4, 12288};
+    case 16: return {mbe_persistent<4, 16384, kMainW, 2>, 4, 16384};
+    case 24: return {mbe_persistent<4, 24576, kMainW, 2>, 4, 24576};
+    default: return {mbe_persistent<4, 8192, kMainW, 2>, 4, 8192};
   }
 }
 
@@ -172,6 +191,9 @@ int Engine::ensure_device(int dev) {
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, dev));
   cu_count = prop.multiProcessorCount;
+  int khz = 0;
+  HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  wall_khz = khz > 0 ? khz : 100000;
   device = dev;
   return GMBE_OK;
 }
@@ -238,7 +260,7 @@ static uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 // Per-wave slab layout for a graph with nV enumeration-side vertices and
 // maximum enumeration-side degree dmaxV.
 mbe::Scratch scratch_layout(uint32_t nV, uint32_t dmaxV, uint64_t arena_bytes,
-                            uint64_t ctx_bytes) {
+                            uint64_t ctx_bytes, uint32_t ctx_slots) {
   mbe::Scratch s{};
   uint64_t o = 0;
   auto take = [&](uint64_t bytes) {
@@ -264,8 +286,9 @@ mbe::Scratch scratch_layout(uint32_t nV, uint32_t dmaxV, uint64_t arena_bytes,
   s.o_fr = take((d + 2) * sizeof(mbe::Frame));
   s.o_arena = take(arena_bytes);
   s.arena_bytes = arena_bytes;
-  s.o_ctx = take(ctx_bytes);
+  s.o_ctx = take(ctx_bytes * ctx_slots);
   s.ctx_bytes = ctx_bytes;
+  s.ctx_slots = ctx_slots;
   s.slab = o;
   return s;
 }
@@ -280,6 +303,11 @@ struct gmbe_finder {
   // knobs
   int64_t waves_per_block = 4, blocks = 0, prune = 1, max_prefix = 4, lds_kb = 8;
   int64_t queue_log2 = 22, list_cap = 0, arena_kb = 16384, root_stride = 1, root_phase = 0;
+  int64_t root_begin = 0, root_end = -1;  // processing-order rank range, -1 = all
+  int64_t deadline_ms = 0;                 // 0 = run to completion
+  int64_t late_share = 32;                 // 0 = off
+  int64_t occupancy = 2;                   // waves per SIMD the kernels are built for
+  int64_t ctx_slots = 4;                   // shared-context slots per wave
   int64_t profile = 0, group = 1, share = 1, ctx_kb = 4096, share_work = 4096, share_queue = -1;
   int64_t merge_min = 256, merge_branch = 16;
   unsigned long long prof[mbe::A_COUNT] = {};
@@ -311,7 +339,7 @@ struct gmbe_finder {
 using namespace gmbe;
 
 static int finder_alloc(gmbe_finder* f) {
-  KernelChoice kc = pick_kernel((int)f->lds_kb);
+  KernelChoice kc = pick_kernel((int)f->lds_kb, (int)f->occupancy);
   int per_sm = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_sm, (const void*)kc.fn, kc.waves * 64,
                                                        0));
@@ -319,7 +347,7 @@ static int finder_alloc(gmbe_finder* f) {
   int blocks = f->blocks > 0 ? (int)f->blocks : per_sm * f->eng.cu_count;
   int nwaves = blocks * kc.waves;
   mbe::Scratch lay = scratch_layout(f->dg.nV, f->dg.dmaxV, (uint64_t)f->arena_kb * 1024ull,
-                                    (uint64_t)f->ctx_kb * 1024ull);
+                                    (uint64_t)f->ctx_kb * 1024ull, (uint32_t)f->ctx_slots);
   uint64_t bytes = lay.slab * (uint64_t)nwaves;
   if (bytes != f->scratch_bytes || nwaves != f->nwaves) {
     if (f->scratch) (void)hipFree(f->scratch);
@@ -430,6 +458,11 @@ int gmbe_finder_set(gmbe_finder* f, const char* key, int64_t value) {
   else if (k == "ctx_kb") { if ((rc = range(16, 1 << 20))) return rc; f->ctx_kb = value; }
   else if (k == "group") { if ((rc = range(0, 1))) return rc; f->group = value; }
   else if (k == "profile") { if ((rc = range(0, 1))) return rc; f->profile = value; }
+  else if (k == "ctx_slots") { if ((rc = range(1, 32))) return rc; f->ctx_slots = value; }
+  else if (k == "late_share") { if ((rc = range(0, 1ll << 30))) return rc; f->late_share = value; }
+  else if (k == "deadline_ms") { if ((rc = range(0, 1ll << 32))) return rc; f->deadline_ms = value; }
+  else if (k == "root_begin") { if ((rc = range(0, (1ll << 32)))) return rc; f->root_begin = value; }
+  else if (k == "root_end") { if ((rc = range(-1, (1ll << 32)))) return rc; f->root_end = value; }
   else if (k == "root_phase") { if ((rc = range(0, (1 << 30) - 1))) return rc; f->root_phase = value; }
   else if (k == "bound_height") { if ((rc = range(0, 1 << 30))) return rc; f->bound_height = (int)value; }
   else if (k == "bound_size") { if ((rc = range(0, (int64_t)1 << 30))) return rc; f->bound_size = (int)value; }
@@ -447,8 +480,12 @@ int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res) {
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(f->stream));
   double t0 = now_s();  // start_time_ (src/IterFinderGpu.cu:1096)
-  uint64_t nV = f->dg.nV, stride = (uint64_t)f->root_stride, phase = (uint64_t)f->root_phase;
-  uint64_t n_roots = nV > phase ? (nV - phase + stride - 1) / stride : 0;
+  // roots: processing ranks r in [root_begin, root_end) with r % stride == phase
+  const uint64_t nV = f->dg.nV, stride = (uint64_t)f->root_stride;
+  const uint64_t end = f->root_end < 0 ? nV : std::min<uint64_t>((uint64_t)f->root_end, nV);
+  const uint64_t b = (uint64_t)f->root_begin;
+  const uint64_t first = b + ((uint64_t)f->root_phase + stride - b % stride) % stride;
+  const uint64_t n_roots = end > first ? (end - first + stride - 1) / stride : 0;
   std::vector<unsigned long long> ctl_h(C_NSLOTS * kCtlStride, 0ull);
   ctl_h[C_WORK_LEFT * kCtlStride] = n_roots;
   HIP_TRY(hipMemcpyAsync(f->ctl, ctl_h.data(), ctl_h.size() * 8, hipMemcpyHostToDevice,
@@ -472,7 +509,7 @@ int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res) {
   P.s.qcap = f->qcap;
   P.s.n_roots = n_roots;
   P.s.stride = (uint32_t)stride;
-  P.s.phase = (uint32_t)phase;
+  P.s.phase = (uint32_t)first;
   P.s.root_tasks = nullptr;
   P.w = f->layout;
   P.acc = f->acc;
@@ -485,12 +522,17 @@ int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res) {
   P.split = f->sel == 2;
   P.profile = (int)f->profile;
   P.group = (int)f->group;
-  P.share = (int)f->share;
+  P.share = f->share && f->sel == 2;  // sharing splits a task: only for -s 2
   P.merge_min = (int)f->merge_min;
   P.merge_branch = (int)f->merge_branch;
+  P.budget = (uint64_t)f->deadline_ms * (uint64_t)f->eng.wall_khz;
+  P.late_share = f->share && f->sel == 2 ? (int)f->late_share : 0;  // a form of splitting
   P.share_work = (int)std::min<int64_t>(f->share_work, INT32_MAX);
   P.share_queue = f->share_queue >= 0 ? (int)f->share_queue : f->nwaves;
-  KernelChoice kc = pick_kernel((int)f->lds_kb);
+  KernelChoice kc = pick_kernel((int)f->lds_kb, (int)f->occupancy);
+  hipLaunchKernelGGL(mbe::reset_ctx, dim3(64), dim3(256), 0, f->stream, P,
+                     (unsigned long long)f->nwaves * (unsigned long long)f->ctx_slots);
+  HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(f->ev0, f->stream));
   if (n_roots > 0) {
     hipLaunchKernelGGL(kc.fn, dim3(f->nblocks), dim3(kc.waves * 64), 0, f->stream, P);
@@ -501,7 +543,12 @@ int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res) {
                          f->stream));
   HIP_TRY(hipStreamSynchronize(f->stream));
   f->last_deferred = std::min<uint64_t>(ctl_out[C_DEFERRED * kCtlStride], f->deferred_cap);
-  if (f->last_deferred > 0) {
+  // the wide pass gets what is left of the budget
+  const double used_ms = (now_s() - t0) * 1e3;
+  const bool out_of_time = f->deadline_ms > 0 && used_ms >= (double)f->deadline_ms;
+  if (f->deadline_ms > 0 && !out_of_time)
+    P.budget = (uint64_t)(((double)f->deadline_ms - used_ms) * f->eng.wall_khz);
+  if (f->last_deferred > 0 && !out_of_time) {
     // wide-bitset pass over the deferred tasks (same queue, fresh counters)
     std::vector<unsigned long long> ctl2(C_NSLOTS * kCtlStride, 0ull);
     ctl2[C_WORK_LEFT * kCtlStride] = f->last_deferred;
@@ -514,7 +561,7 @@ int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res) {
     P.s.n_roots = f->last_deferred;
     P.s.deferred = nullptr;
     P.s.deferred_cap = 0;
-    KernelChoice wk = pick_kernel(8, true);
+    KernelChoice wk = pick_kernel(8, 1, true);
     hipLaunchKernelGGL(wk.fn, dim3(f->nblocks), dim3(wk.waves * 64), 0, f->stream, P);
     HIP_TRY(hipGetLastError());
   }
@@ -542,6 +589,15 @@ int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res) {
   f->list_used = std::min<uint64_t>(ctl_out[C_LIST_CURSOR * kCtlStride], f->list_words);
   f->list_ovf = acc_h[mbe::A_LIST_OVF];
   memcpy(f->prof, acc_h, sizeof(acc_h));
+  const uint64_t roots_claimed = std::min<uint64_t>(ctl_out[C_ROOT_NEXT * kCtlStride], n_roots);
+  if (f->deadline_ms > 0 &&
+      (acc_h[mbe::A_ABANDONED] > 0 || ctl_out[C_WORK_LEFT * kCtlStride] != 0 ||
+       (f->last_deferred > 0 && out_of_time)))
+    return set_error(GMBE_ERR_DEADLINE,
+                     "deadline of %lld ms passed: %llu of %llu top-level vertices claimed, %llu "
+                     "subtree(s) abandoned; the counts are partial",
+                     (long long)f->deadline_ms, (unsigned long long)roots_claimed,
+                     (unsigned long long)n_roots, (unsigned long long)acc_h[mbe::A_ABANDONED]);
   if (res->overflow)
     return set_error(GMBE_ERR_CAPACITY,
                      "%llu task(s) exceeded a capacity (%llu list arena (arena_kb=%lld), %llu |L| > "

@@ -68,7 +68,8 @@ enum Acc : int {
   // profile (filled when Params::profile is set): cycles per phase, work sizes
   A_CYC_WAIT, A_CYC_PREFIX, A_CYC_EXPAND, A_CYC_EXTRACT, A_CYC_BUILD, A_CYC_DFS, A_CYC_SPLIT,
   A_EDGES, A_ENTRIES, A_OVF_ARENA, A_OVF_WIDE, A_OVF_DEFER, A_ENT_W1, A_NODES_W1, A_CYC_TAIL,
-  A_CYC_PASS, A_CYC_MERGE, A_CYC_COMPRESS, A_CYC_PUBLISH, A_PUBLISHES, A_MERGES, A_COUNT
+  A_CYC_PASS, A_CYC_MERGE, A_CYC_COMPRESS, A_CYC_PUBLISH, A_PUBLISHES, A_MERGES, A_ABANDONED,
+  A_LATE, A_COUNT
 };
 
 struct Sched {
@@ -80,7 +81,7 @@ struct Sched {
   unsigned long long* queued;     // entries pushed and not yet popped (a hint)
   unsigned long long qcap;        // slots per shard, power of two
   unsigned long long n_roots;     // top-level vertices of this run (sample)
-  uint32_t stride, phase;         // root r -> vertex nV-1-(r*stride+phase)
+  uint32_t stride, phase;         // root r -> vertex nV-1-(r*stride+phase) (phase = first rank)
   const Task* root_tasks;         // if set, root r is root_tasks[r] instead
   Task* deferred;                 // tasks too wide for this kernel's bitsets
   unsigned long long* n_deferred;
@@ -93,7 +94,8 @@ struct Scratch {
   // offsets inside a slab
   uint64_t o_cmap, o_bm0, o_bm1, o_L, o_Lp, o_T, o_kL, o_U, o_Ucnt, o_C, o_kC, o_R, o_fr,
       o_arena, o_ctx;
-  uint64_t arena_bytes, ctx_bytes;
+  uint64_t arena_bytes, ctx_bytes;  // ctx_bytes: per context slot
+  uint32_t ctx_slots;                // context slots per wave
   uint32_t nW0, nW1;
 };
 
@@ -113,6 +115,9 @@ struct Params {
   int share_queue;           // share nodes only while fewer queue entries wait
   int merge_min;             // merge twins of child lists at least this long
   int merge_branch;          // ... with at least this many pushable entries
+  int late_share;            // min list length of a frame handed over late (0 = off)
+  uint64_t budget;           // wall-clock ticks a wave may run (0 = no deadline)
+  uint64_t deadline;         // set per wave: its start + budget
 };
 
 struct Counters {
@@ -124,7 +129,15 @@ struct Counters {
   uint64_t ent_w1 = 0, nodes_w1 = 0;  // the part of entries / pushes with one-word bitsets
   uint64_t tail = 0;                   // cycles of the final wait for work
   uint64_t c_pass = 0, c_merge = 0, c_compress = 0, c_publish = 0, publishes = 0, merges = 0;
+  uint64_t abandoned = 0;              // subtrees left unfinished at the deadline
+  uint64_t late = 0;                   // frames handed over by late sharing
 };
+
+// True once this wave's deadline has passed (never without a budget).  The
+// constant-rate wall clock is read per wave; no memory is touched.
+__device__ __forceinline__ bool past_deadline(const Params& P) {
+  return P.budget != 0 && wall_clock64() > P.deadline;
+}
 
 __device__ __forceinline__ uint64_t cycles() { return __builtin_readcyclecounter(); }
 // Adds the cycles since `t0` to phase `ph` (when profiling) and restarts t0.
@@ -389,6 +402,7 @@ __device__ int next_work(const Params& P, uint32_t& word, Task& t, int& k, uint3
   uint32_t lane = lane_id();
   uint32_t nap = 1;
   for (;;) {
+    if (past_deadline(P)) return 0;  // leave what is queued: the run is cut short
     if (queue_pop(s, word, home)) return 1;
     unsigned long long r = ~0ull;
     if (lane == 0 && ld_dev(s.root_next) < s.n_roots) {
@@ -706,8 +720,21 @@ struct CtxHdr {
   uint32_t pad[9];
 };
 
-__device__ __forceinline__ CtxHdr* ctx_hdr(const Params& P, uint64_t wave) {
-  return (CtxHdr*)(P.w.base + wave * P.w.slab + P.w.o_ctx);
+// Context id = wave * ctx_slots + slot; a wave publishes only into its own
+// slots, and a slot is free again once its refs count has dropped to zero.
+__device__ __forceinline__ CtxHdr* ctx_hdr(const Params& P, uint64_t id) {
+  const uint64_t wave = id / P.w.ctx_slots, slot = id % P.w.ctx_slots;
+  return (CtxHdr*)(P.w.base + wave * P.w.slab + P.w.o_ctx + slot * P.w.ctx_bytes);
+}
+
+// A free context slot of wave gw (its id), or kNone.  Lane s polls slot s.
+__device__ __forceinline__ uint32_t free_ctx(const Params& P, uint64_t gw) {
+  const uint32_t lane = lane_id();
+  const uint64_t id = gw * P.w.ctx_slots + lane;
+  bool free_slot = false;
+  if (lane < P.w.ctx_slots) free_slot = ld_dev(&ctx_hdr(P, id)->refs) == 0ull;
+  const uint64_t b = ballot(free_slot);
+  return b ? (uint32_t)(gw * P.w.ctx_slots) + (uint32_t)__builtin_ctzll(b) : kNone;
 }
 // context layout after the header: list (len recs), kL (nL), C (nC)
 template <int W>
@@ -731,15 +758,19 @@ constexpr int kItemsPerEntry = 15;  // (position, candidate) pairs after hdr, ow
 // Publish a node's list `root` (len entries, in this wave's arenas) as this
 // wave's context and push one item per pushable entry.  The task arrays
 // `src` (local or remote) are copied so that the context is self-contained.
-// Returns false (nothing published) when the queue lacks room or the node has
-// nothing to push.
+// Only entries whose candidate is at least min_rep become items: a node that
+// has already pushed its first candidates publishes the rest.  Returns false
+// (nothing published) when the queue lacks room or fewer than min_items
+// entries are left to push.
 template <int W, bool KL_R, bool C_R>
-__device__ bool publish_list(const Params& P, uint64_t gw, const Rec<W>* root, uint32_t len,
+__device__ bool publish_list(const Params& P, uint64_t gw, uint32_t ctx, const Rec<W>* root,
+                             uint32_t len,
                              uint32_t nL, uint32_t nC, uint32_t nbits, uint64_t sumR0,
-                             const TaskArrays& src) {
+                             const TaskArrays& src, uint32_t min_rep = 0,
+                             uint32_t min_items = 1) {
   const uint32_t lane = lane_id();
   const Sched& s = P.s;
-  CtxHdr* h = ctx_hdr(P, gw);
+  CtxHdr* h = ctx_hdr(P, ctx);
   Rec<W>* recs;
   TaskArrays ta;
   ctx_arrays<W>(h, nL, nC, len, recs, ta);
@@ -747,10 +778,14 @@ __device__ bool publish_list(const Params& P, uint64_t gw, const Rec<W>* root, u
   uint32_t n_items = 0;
   for (uint32_t base = 0; base < len; base += kWave) {
     uint32_t i = base + lane;
-    bool it = i < len && root[i].rep != kNone;
+    bool it = false;
+    if (i < len) {
+      const uint32_t r = root[i].rep;
+      it = r != kNone && r >= min_rep;
+    }
     n_items += __popcll(ballot(it));
   }
-  if (n_items == 0) return false;
+  if (n_items == 0 || n_items < min_items) return false;
   const uint32_t ne = (n_items + kItemsPerEntry - 1) / kItemsPerEntry;
   // copy the context with device-scope stores
   for (uint32_t i = lane; i < len; i += kWave) {
@@ -788,7 +823,11 @@ __device__ bool publish_list(const Params& P, uint64_t gw, const Rec<W>* root, u
   uint32_t j0 = 0;
   for (uint32_t base = 0; base < len; base += kWave) {
     uint32_t i = base + lane;
-    bool it = i < len && root[i].rep != kNone;
+    bool it = false;
+    if (i < len) {
+      const uint32_t r = root[i].rep;
+      it = r != kNone && r >= min_rep;
+    }
     uint64_t bm = ballot(it);
     if (it) {
       uint32_t j = j0 + lanes_below(bm);
@@ -800,7 +839,7 @@ __device__ bool publish_list(const Params& P, uint64_t gw, const Rec<W>* root, u
         while (ld_dev(s.qseq + slot) != pos) __builtin_amdgcn_s_sleep(1);
         uint32_t cnt = n_items - j < (uint32_t)kItemsPerEntry ? n_items - j : kItemsPerEntry;
         st_dev(e + 0, kItemKind | cnt);
-        st_dev(e + 1, (uint32_t)gw);
+        st_dev(e + 1, ctx);
       }
     }
     wave_drain();
@@ -926,18 +965,18 @@ __device__ uint32_t merge_twins(const WaveMem& m, Rec<W>* crec, uint32_t len, ui
   return n;
 }
 
-// Whether sharing a node is worthwhile now: this wave's context area is free
-// and large enough, the top-level vertices are all claimed and the queue is
-// short.
-__device__ __forceinline__ bool share_wanted(const Params& P, uint64_t gw, uint32_t nL, uint32_t nC,
-                                             uint32_t len, int W) {
+// Whether sharing a node is worthwhile now: the top-level vertices are all
+// claimed, the queue is short and one of this wave's context slots is free
+// and large enough.  Returns that slot's id, or kNone.
+__device__ __forceinline__ uint32_t share_wanted(const Params& P, uint64_t gw, uint32_t nL,
+                                                 uint32_t nC, uint32_t len, int W) {
   const uint64_t bytes = sizeof(CtxHdr) + (uint64_t)len * (16 + 8 * W) + (uint64_t)nC * 4 +
                          (uint64_t)nL * 8;
-  if (bytes > P.w.ctx_bytes) return false;
-  if (uni64(ld_dev(&ctx_hdr(P, gw)->refs)) != 0ull) return false;
-  if (uni64(ld_dev(P.s.root_next)) < P.s.n_roots) return false;
+  if (bytes > P.w.ctx_bytes) return kNone;
+  if (uni64(ld_dev(P.s.root_next)) < P.s.n_roots) return kNone;
   const long long depth = (long long)uni64(ld_dev(P.s.queued));
-  return depth < (long long)P.share_queue;
+  if (depth >= (long long)P.share_queue) return kNone;
+  return free_ctx(P, gw);
 }
 
 template <int W, bool KL_R, bool C_R>
@@ -998,10 +1037,12 @@ __device__ bool child_node(const Params& P, const WaveMem& m, uint64_t gw, Count
         }
         TaskArrays tc{kLc, ta.C};
         bool shared = false;
-        if (P.share && (uint64_t)dlen * dcb >= (uint64_t)P.share_work &&
-            share_wanted(P, gw, nb, nC, dlen, 1)) {
+        const uint32_t cx = P.share && (uint64_t)dlen * dcb >= (uint64_t)P.share_work
+                                ? share_wanted(P, gw, nb, nC, dlen, 1)
+                                : kNone;
+        if (cx != kNone) {
           uint64_t tq = P.profile ? cycles() : 0;
-          shared = publish_list<1, false, C_R>(P, gw, dst, dlen, nb, nC, nb, sumR, tc);
+          shared = publish_list<1, false, C_R>(P, gw, cx, dst, dlen, nb, nC, nb, sumR, tc);
           if (P.profile) ctr.c_publish += cycles() - tq;
         }
         if (shared) {
@@ -1036,10 +1077,12 @@ __device__ bool child_node(const Params& P, const WaveMem& m, uint64_t gw, Count
     if (P.profile) ctr.c_merge += cycles() - tm;
     ctr.merges++;
   }
-  if (P.share && !P.list && (uint64_t)clen * cbranch >= (uint64_t)P.share_work &&
-      share_wanted(P, gw, nL, nC, clen, W)) {
+  const uint32_t cx = P.share && !P.list && (uint64_t)clen * cbranch >= (uint64_t)P.share_work
+                          ? share_wanted(P, gw, nL, nC, clen, W)
+                          : kNone;
+  if (cx != kNone) {
     uint64_t tq = P.profile ? cycles() : 0;
-    bool ok = publish_list<W, KL_R, C_R>(P, gw, crec, clen, nL, nC, nb, sumR, ta);
+    bool ok = publish_list<W, KL_R, C_R>(P, gw, cx, crec, clen, nL, nC, nb, sumR, ta);
     if (P.profile) ctr.c_publish += cycles() - tq;
     if (ok) {
       ctr.splits++;
@@ -1048,6 +1091,59 @@ __device__ bool child_node(const Params& P, const WaveMem& m, uint64_t gw, Count
     }
   }
   return true;
+}
+
+// Late sharing: when other waves run dry while this one is deep in a large
+// subtree, hand the unpushed rest of its shallowest open frame to them.  The
+// frame's remaining pushes are exactly its entries with candidate >= its next
+// candidate (pushes go by increasing candidate and pruned entries are marked
+// unpushable in place), so publishing that suffix and closing the frame keeps
+// the traversal's result.  Returns true when the current frame was closed.
+template <int W, bool KL_R, bool C_R>
+__device__ bool late_share(const Params& P, uint64_t gw, uint32_t nL, uint32_t nC,
+                           const TaskArrays& ta, Frame& F, Frame* fr, uint32_t depth,
+                           Counters& ctr) {
+  // cheap gate first: the top-level vertices are all claimed and the queue is
+  // short; then pick the shallowest frame with pushes left
+  if (uni64(ld_dev(P.s.root_next)) < P.s.n_roots) return false;
+  if ((long long)uni64(ld_dev(P.s.queued)) >= (long long)P.share_queue) return false;
+  uint32_t at = kNone;
+  Frame S;
+  const uint32_t scan = depth < 4u ? depth : 4u;
+  for (uint32_t i = 0; i < scan; ++i) {
+    const uint32_t cur = uni(fr[i].cur);
+    if (cur != kNone) {
+      at = i;
+      S.rec = uni64(fr[i].rec);
+      S.len = uni(fr[i].len);
+      S.curidx = uni(fr[i].curidx);
+      S.nbits = uni(fr[i].nbits);
+      S.sumR = uni64(fr[i].sumR);
+      break;
+    }
+  }
+  if (at == kNone) {
+    if (depth > 4u) return false;
+    S = F;
+  }
+  if (S.len < (uint32_t)P.late_share) return false;
+  const uint32_t cx = share_wanted(P, gw, nL, nC, S.len, W);
+  if (cx == kNone) return false;
+  uint64_t tq = P.profile ? cycles() : 0;
+  const bool ok = publish_list<W, KL_R, C_R>(P, gw, cx, (const Rec<W>*)S.rec, S.len, nL, nC,
+                                             S.nbits, S.sumR, ta, S.curidx, 2);
+  if (P.profile) ctr.c_publish += cycles() - tq;
+  if (!ok) return false;
+  ctr.splits++;
+  ctr.publishes++;
+  ctr.late++;
+  if (at == kNone) {
+    F.cur = kNone;
+    return true;
+  }
+  if (lane_id() == 0) fr[at].cur = kNone;
+  wave_mem_wait();
+  return false;
 }
 
 // The stack machine from frame F (a list in this wave's arenas); saved
@@ -1059,8 +1155,13 @@ __device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC,
   typedef Rec<W> R_t;
   const uint32_t lane = lane_id();
   const bool prune = P.prune != 0;
-  uint32_t depth = 0;
+  uint32_t depth = 0, tick = 0;
   for (;;) {
+    ++tick;
+    if (P.budget != 0 && (tick & 63u) == 0 && past_deadline(P)) {
+      ctr.abandoned++;
+      return;
+    }
     if (F.cur == kNone) {
       // frame exhausted: free its list, return to the parent
       if (F.tier == 0) lds_top = F.top; else hbm_top = F.top;
@@ -1078,6 +1179,9 @@ __device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC,
       F.top = uni(F.top);
       continue;
     }
+    if (P.late_share && !P.list && (tick & 7u) == 0 &&
+        late_share<W, KL_R, C_R>(P, gw, nL, nC, ta, F, fr, depth, ctr))
+      continue;  // the current frame was handed over: F.cur is now kNone
     R_t* recs = (R_t*)F.rec;
     const uint32_t xpos = F.cur, xidx = F.curidx;
     {
@@ -1306,7 +1410,7 @@ __device__ void run_items(const Params& P, const WaveMem& m, uint32_t word, Coun
 template <int Wt>
 __device__ void enumerate_task(const Params& P, const WaveMem& m, const uint32_t* L, uint32_t nL,
                                uint32_t nC, uint32_t next_id, uint64_t sumR, uint32_t nR,
-                               Counters& ctr, uint64_t& t0, bool share, uint64_t gw) {
+                               Counters& ctr, uint64_t& t0, uint32_t share_ctx, uint64_t gw) {
   typedef Rec<Wt> R_t;
   const DevGraph& g = P.g;
   const uint32_t lane = lane_id();
@@ -1349,8 +1453,9 @@ __device__ void enumerate_task(const Params& P, const WaveMem& m, const uint32_t
   }
   wave_mem_wait();
   TaskArrays lta{m.kL, m.C};
-  if (share && publish_list<Wt, false, false>(P, gw, (const R_t*)list_addr, len, nL, nC, nL, sumR,
-                                              lta)) {
+  if (share_ctx != kNone &&
+      publish_list<Wt, false, false>(P, gw, share_ctx, (const R_t*)list_addr, len, nL, nC, nL,
+                                     sumR, lta)) {
     ctr.splits++;
     phase(ctr, P.profile, 6, t0);
     return;
@@ -1594,15 +1699,15 @@ __device__ void run_task(const Params& P, const WaveMem& m, const Task& t, int k
   bool rule = P.split && ((rest > (uint32_t)P.bound_height || nL > (uint32_t)P.bound_height) &&
                           (uint64_t)rest * nL > (uint64_t)P.bound_size);
   bool must = W > (uint32_t)kMaxW;  // |L| beyond any bitset: split regardless
-  // large task: share its root list with other waves when this wave's context
-  // area is free and large enough
-  bool share = rule && P.share && !P.list && W <= (uint32_t)WMAX &&
-               ld_dev(&ctx_hdr(P, gw)->refs) == 0ull;
-  if (share) {
+  // large task: share its root list with other waves when one of this wave's
+  // context slots is free and large enough
+  uint32_t share_ctx = kNone;
+  if (rule && P.share && !P.list && W <= (uint32_t)WMAX) {
     const uint32_t Wt = W <= 1 ? 1 : (W <= 2 ? 2 : (W <= 4 ? 4 : (W <= 8 ? 8 : 16)));
     const uint64_t bytes = sizeof(CtxHdr) + (uint64_t)nC * (16 + 8 * Wt + 12) + (uint64_t)nL * 8;
-    share = bytes <= P.w.ctx_bytes;
+    if (bytes <= P.w.ctx_bytes) share_ctx = free_ctx(P, gw);
   }
+  const bool share = share_ctx != kNone;
   if (!share && ((rule && k < P.max_prefix) || (must && k < kMaxPrefix))) {
     // children S + [C[i]], i >= next_id, each recomputed by its wave
     const unsigned long long ne = (rest + kChunk - 1) / kChunk;
@@ -1660,14 +1765,14 @@ __device__ void run_task(const Params& P, const WaveMem& m, const Task& t, int k
     if (P.list) m.T[j] = L[j];
   }
   if (W <= 1) {
-    enumerate_task<1>(P, m, L, nL, nC, next_id, sumR, nR, ctr, t0, share, gw);
+    enumerate_task<1>(P, m, L, nL, nC, next_id, sumR, nR, ctr, t0, share_ctx, gw);
   } else if (W <= 2) {
-    enumerate_task<2>(P, m, L, nL, nC, next_id, sumR, nR, ctr, t0, share, gw);
+    enumerate_task<2>(P, m, L, nL, nC, next_id, sumR, nR, ctr, t0, share_ctx, gw);
   } else if (W <= 4) {
-    enumerate_task<4>(P, m, L, nL, nC, next_id, sumR, nR, ctr, t0, share, gw);
+    enumerate_task<4>(P, m, L, nL, nC, next_id, sumR, nR, ctr, t0, share_ctx, gw);
   } else if constexpr (WMAX >= 16) {
-    if (W <= 8) enumerate_task<8>(P, m, L, nL, nC, next_id, sumR, nR, ctr, t0, share, gw);
-    else enumerate_task<16>(P, m, L, nL, nC, next_id, sumR, nR, ctr, t0, share, gw);
+    if (W <= 8) enumerate_task<8>(P, m, L, nL, nC, next_id, sumR, nR, ctr, t0, share_ctx, gw);
+    else enumerate_task<16>(P, m, L, nL, nC, next_id, sumR, nR, ctr, t0, share_ctx, gw);
   }
 }
 

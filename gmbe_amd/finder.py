@@ -35,8 +35,9 @@ class IterFinderGpu6:
 
     def execute(self) -> Result:
         r = Result()
-        check(lib().gmbe_finder_execute(self._h, C.byref(r)))
-        self.result = r
+        rc = lib().gmbe_finder_execute(self._h, C.byref(r))
+        self.result = r  # partial counts stay readable after GMBE_ERR_DEADLINE
+        check(rc)
         return r
 
     def bicliques(self):
@@ -51,7 +52,7 @@ class IterFinderGpu6:
     STAT_NAMES = ["cyc_wait", "cyc_prefix", "cyc_expand", "cyc_extract", "cyc_build", "cyc_dfs",
                   "cyc_split", "edges", "entries", "ovf_arena", "ovf_wide", "ovf_defer",
                   "entries_w1", "nodes_w1", "cyc_tail", "sub_pass", "sub_merge",
-                  "sub_compress", "sub_publish", "publishes", "merges"]
+                  "sub_compress", "sub_publish", "publishes", "merges", "abandoned", "late"]
 
     def stats(self) -> dict:
         out = (C.c_uint64 * 32)()

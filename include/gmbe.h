@@ -39,6 +39,7 @@ extern "C" {
 #define GMBE_ERR_REJECTED (-7)    /* serving admission rejected a request   */
 #define GMBE_ERR_UNSUPPORTED (-8) /* option not supported on this build     */
 #define GMBE_ERR_STATE (-9)       /* call out of order (e.g. list before run) */
+#define GMBE_ERR_DEADLINE (-10)   /* deadline_ms passed; counts are partial  */
 
 typedef struct gmbe_graph gmbe_graph;
 typedef struct gmbe_finder gmbe_finder;
@@ -173,8 +174,19 @@ int gmbe_finder_create(const gmbe_graph* g, int sel, int bound_height,
 
 /* Tuning knobs, set before execute (the reference's compile-time macros,
  * src/CMakeLists.txt:13-39, src/GpuUtil.cuh:9-24, src/IterFinderGpu.cu:912-917):
- *   "waves_per_block", "blocks", "queue_cap", "list_cap" (u32 words of list
- *   output; 0 = off), "prune" (0/1, PRUNE_EN), "max_prefix" (2..8). */
+ *   "waves_per_block", "blocks", "queue_log2", "list_cap" (u32 words of list
+ *   output; 0 = off), "prune" (0/1, PRUNE_EN), "max_prefix" (2..8),
+ *   "bound_height", "bound_size".
+ * Scheduler and memory knobs (no counterpart): "lds_kb", "arena_kb",
+ *   "ctx_kb", "ctx_slots", "group", "share", "share_work", "share_queue", "merge_min",
+ *   "merge_branch", "late_share", "profile".
+ * "deadline_ms" (0 = none): every wave stops taking work, and abandons the
+ *   subtree it is in, once it has run that long; execute then returns
+ *   GMBE_ERR_DEADLINE with the partial counts filled in.
+ * Root selection: the run enumerates the tasks of the top-level vertices whose
+ *   processing rank r satisfies root_begin <= r < root_end (-1 = all) and
+ *   r % root_stride == root_phase; disjoint selections partition the result
+ *   (counts and hash sums add, hash xors xor). */
 int gmbe_finder_set(gmbe_finder* f, const char* key, int64_t value);
 
 /* Replaces IterFinderGpu6::Execute (src/IterFinderGpu.cu:1095-1119):

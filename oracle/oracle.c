@@ -34,6 +34,12 @@
  *   h = mix64(sum_A key(0,a) ^ rotl64(sum_B key(1,b), 31))
  * with mix64 the splitmix64 output function.  Digest = (count, sum h, xor h).
  *
+ * Sampling and time limits: every enumerator takes the task roots of
+ * processing ranks phase, phase + stride, ... (phase may exceed the stride).
+ * time_limit stops claiming roots once passed; orc_mbea also abandons the
+ * subtrees it is in, so its counts are then partial but its rate is that of
+ * the work done (the benchmark's CPU baseline reads it that way).
+ *
  * Parity pinning of each function: see tests/test_oracle.py (closed forms,
  * brute force, the three-way agreement, and the counts of the reference's
  * own CPU code built from baselines/MBE.zip into oracle/_ref/).
@@ -596,6 +602,9 @@ typedef struct {
   size_t arena_n, arena_cap;
   orc_result res;
   orc_list* list;
+  double deadline; /* 0 = none; past it, subtrees are abandoned (counts partial) */
+  uint64_t tick;
+  int cut;
 } mb_ws;
 
 static uint32_t* mb_alloc(mb_ws* w, size_t n) {
@@ -609,6 +618,8 @@ static uint32_t* mb_alloc(mb_ws* w, size_t n) {
 static int mb_find(mb_ws* w, const uint32_t* L, int64_t nL, const uint32_t* R, int64_t nR,
                    uint64_t sumR, const uint32_t* C, int64_t nC, uint32_t vc) {
   const og* g = w->g;
+  if (w->deadline > 0 && (++w->tick & 255u) == 0 && wall() > w->deadline) w->cut = 1;
+  if (w->cut) return 0;
   size_t mark = w->arena_n;
   uint32_t* Lp = mb_alloc(w, (size_t)nL + 1);
   if (!Lp) return -1;
@@ -763,6 +774,7 @@ static void* run_worker(void* arg) {
     w.touched = malloc(((size_t)g->nV + 1) * 4);
     w.arena_cap = (size_t)1 << 26; /* 256 MB of u32 per thread */
     w.arena = malloc(w.arena_cap * 4);
+    w.deadline = j->deadline;
     if (j->want_list) w.list = orc_list_new();
     if (!w.cmap || !w.touched || !w.arena) {
       wk->rc = -1;
@@ -793,7 +805,9 @@ static int run(int kind, uint32_t nU, uint32_t nV, const int64_t* v_off, const u
                const uint32_t* u_orig, const uint32_t* v_orig, int swapped, gm_params gp,
                int nthreads, uint32_t stride, uint32_t phase, double time_limit, orc_result* out,
                orc_list* list) {
-  if (!out || !v_off || nthreads < 1 || stride < 1 || phase >= stride) return -1;
+  /* roots: processing ranks phase, phase + stride, ... (phase may exceed the
+   * stride, which starts the sample at a later rank) */
+  if (!out || !v_off || nthreads < 1 || stride < 1) return -1;
   memset(out, 0, sizeof(*out));
   og g;
   int rc = og_init(&g, nU, nV, v_off, v_adj, u_orig, v_orig, swapped);

@@ -163,6 +163,41 @@ def test_full_size_config2_root_sample(gpu, stride, phase):
     assert r.digest() == res.digest()
 
 
+def test_root_ranges_partition_the_result(gpu):
+    """Rank ranges, alone and combined with a stride, partition the roots."""
+    g = workloads.run_graph(2, 0.02)
+    full = gpu_digest(g).digest()
+    nV = g.stats()["V_size"]
+    cuts = [0, 7, 500, nV // 2, nV]
+    parts = [gpu_digest(g, root_begin=a, root_end=b).digest() for a, b in zip(cuts, cuts[1:])]
+    parts += [gpu_digest(g, root_begin=a, root_end=b, root_stride=3, root_phase=p).digest()
+              for a, b in [(0, 1000), (1000, nV)] for p in range(3)]
+    for group in (parts[:4], parts[4:]):
+        cnt = sum(d[0] for d in group)
+        hs = sum(d[1] for d in group) % (1 << 64)
+        hx = 0
+        for d in group:
+            hx ^= d[2]
+        assert (cnt, hs, hx) == full
+
+
+def test_deadline_cuts_the_run_short(gpu):
+    """A deadline ends the kernel with partial counts and GMBE_ERR_DEADLINE;
+    a generous one leaves the result exact."""
+    g = workloads.run_graph(2, 0.1)
+    f = IterFinderGpu6(g, deadline_ms=50)
+    with pytest.raises(GmbeError) as e:
+        f.execute()
+    assert e.value.code == -10
+    part = f.result
+    assert 0 < part.kernel_time_s < 5.0
+    assert part.maximal > 0
+    f.close()
+    small = workloads.run_graph(2, 0.01)
+    exact = oracle.gmbe(small.oracle_graph(), threads=8).digest()
+    assert gpu_digest(small, deadline_ms=60000).digest() == exact
+
+
 @pytest.mark.parametrize("knobs", [
     dict(share=0), dict(share=1), dict(group=0), dict(group=1, merge_min=2),
     dict(share=1, share_work=1, share_queue=1 << 30),       # share every node it can
