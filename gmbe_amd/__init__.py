@@ -7,6 +7,7 @@ CSRBiGraph / IterFinderGpu6 surface.
 from ._lib import GmbeError, GenParams, Option, Result, lib  # noqa: F401
 from .graph import BiGraph, make_option  # noqa: F401
 from .finder import IterFinderGpu6, enumerate_graph, records_to_set  # noqa: F401
+from .batch import Batch, Server  # noqa: F401
 
 
 def hash_biclique(A, B) -> int:

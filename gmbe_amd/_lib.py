@@ -85,6 +85,7 @@ SIGNATURES = {
     "gmbe_finder_destroy": (None, [_P]),
     "gmbe_batch_create": (C.c_int, [_PP, C.c_int, C.c_int, C.c_int, _PP]),
     "gmbe_batch_set": (C.c_int, [_P, C.c_char_p, C.c_int64]),
+    "gmbe_batch_get": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int64)]),
     "gmbe_batch_execute": (C.c_int, [_P, C.POINTER(Result), C.POINTER(Result)]),
     "gmbe_batch_destroy": (None, [_P]),
     "gmbe_serve_create": (C.c_int, [_P, C.c_int, C.c_uint64, C.c_int, C.c_int, _PP]),

@@ -46,26 +46,35 @@ void mbe_persistent(Params P) {
       break;
     }
     phase(c, P.profile, 0, tw);
+    uint32_t gid;
     if (kind == 2) {
+      gid = t.gid;
+      m.g = P.gt + gid;
       Prefix pre{nullptr, 0, 0};
-      if (k > 1) pre = prefix_common(P.g, m, t, k - 1);
+      if (k > 1) pre = prefix_common(*m.g, m, t, k - 1);
       run_task<WMAX>(P, m, t, k, pre, c, gw);
     } else if (read_lane(word, 0) & kItemKind) {
-      // items of a shared task context
+      // items of a shared task context (its header names the graph)
+      gid = uni(ld_dev(&ctx_hdr(P, read_lane(word, 1))->gid));
+      m.g = P.gt + gid;
       run_items<WMAX>(P, m, word, c, gw);
     } else {
       // an entry: children prefix + [c_i] of one split task; the prefix's
       // common neighbourhood is computed once for all of them
       const uint32_t hdr = read_lane(word, 0);
-      const int kp = (int)(hdr & 0xffu);
-      const uint32_t cnt = hdr >> 8;
+      const int kp = (int)child_hdr_k(hdr);
+      const uint32_t cnt = child_hdr_cnt(hdr);
+      gid = child_hdr_gid(hdr);
+      m.g = P.gt + gid;
       for (int i = 0; i < kMaxPrefix; ++i) t.v[i] = i < kp ? read_lane(word, 1 + i) : kNone;
-      const Prefix pre = prefix_common(P.g, m, t, kp);
+      t.gid = gid;
+      const Prefix pre = prefix_common(*m.g, m, t, kp);
       for (uint32_t i = 0; i < cnt; ++i) {
         t.v[kp] = read_lane(word, 8 + i);
         run_task<WMAX>(P, m, t, kp + 1, pre, c, gw);
       }
     }
+    if (P.per_graph) flush_graph(P, c, gid);
     work_done(P);
   }
   if (lane_id() == 0) {
@@ -138,6 +147,35 @@ __global__ void init_queue(unsigned long long* seq, unsigned long long cap) {
     seq[i] = i & (cap - 1);
 }
 
+// Fill n u64 words with v.
+__global__ void fill_u64(unsigned long long* p, unsigned long long n, unsigned long long v) {
+  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (unsigned long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Control block of a launch: everything zero but work_left.
+__global__ void ctl_start(unsigned long long* ctl, unsigned long long words, unsigned long long wl,
+                          unsigned long long n_roots) {
+  for (unsigned long long i = threadIdx.x; i < words; i += blockDim.x)
+    ctl[i] = i == wl ? n_roots : 0ull;
+}
+
+// Control block of the wide pass, from the deferred count the main pass left.
+__global__ void ctl_wide(unsigned long long* ctl, unsigned long long words, unsigned long long wl,
+                         unsigned long long nd, unsigned long long wn, unsigned long long cur,
+                         unsigned long long cap) {
+  const unsigned long long n = ctl[nd] < cap ? ctl[nd] : cap;
+  const unsigned long long cursor = ctl[cur];
+  __syncthreads();
+  for (unsigned long long i = threadIdx.x; i < words; i += blockDim.x) {
+    unsigned long long v = 0ull;
+    if (i == wl || i == wn) v = n;
+    if (i == cur) v = cursor;
+    ctl[i] = v;
+  }
+}
+
 }  // namespace mbe
 
 namespace gmbe {
@@ -150,35 +188,69 @@ namespace gmbe {
                        __FILE__, __LINE__);                                                  \
   } while (0)
 
-typedef void (*KernelFn)(mbe::Params);
-
-// control counters live kCtlStride u64 (256 bytes) apart so that polling one
-// never contends with atomics on another; the shards' head/tail pairs follow
 using mbe::kCtlStride;
 using mbe::kShards;
-enum CtlSlot { C_ROOT_NEXT = 0, C_WORK_LEFT, C_QUEUED, C_LIST_CURSOR, C_DEFERRED, C_QCTL = 8,
-               C_NSLOTS = 8 + 2 * kShards };
 
-struct KernelChoice {
-  KernelFn fn;
-  int waves;
-  int lds_per_wave;
-};
-
-static KernelChoice pick_kernel(int lds_kb, int occ, bool wide = false) {
+KernelChoice pick_kernel(int lds_kb, int waves, bool wide) {
   using namespace mbe;
   // every kernel shares the stack-machine functions, whose registers are
   // allocated for the laxest caller: all of them ask for two waves per SIMD
-  (void)occ;
   if (wide) return {mbe_persistent<4, 8192, kMaxW, 2>, 4, 8192};
+  if (waves == 8 && lds_kb == 4) return {mbe_persistent<8, 4096, kMainW, 2>, 8, 4096};
+  if (waves == 2 && lds_kb == 16) return {mbe_persistent<2, 16384, kMainW, 2>, 2, 16384};
+  if (waves != 4) return {nullptr, 0, 0};
   switch (lds_kb) {
     case 0: return {mbe_persistent<4, 0, kMainW, 2>, 4, 0};
     case 4: return {mbe_persistent<4, 4096, kMainW, 2>, 4, 4096};
+    case 8: return {mbe_persistent<4, 8192, kMainW, 2>, 4, 8192};
     case 12: return {mbe_persistent<4, 12288, kMainW, 2>, 4, 12288};
     case 16: return {mbe_persistent<4, 16384, kMainW, 2>, 4, 16384};
     case 24: return {mbe_persistent<4, 24576, kMainW, 2>, 4, 24576};
-    default: return {mbe_persistent<4, 8192, kMainW, 2>, 4, 8192};
+    default: return {nullptr, 0, 0};
   }
+}
+
+int resident_blocks(const KernelChoice& kc, int cu_count) {
+  int per_sm = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_sm, (const void*)kc.fn, kc.waves * 64,
+                                                   0) != hipSuccess ||
+      per_sm < 1)
+    per_sm = 1;
+  return per_sm * cu_count;
+}
+
+hipError_t launch_persistent(const KernelChoice& kc, int blocks, hipStream_t st,
+                             const mbe::Params& P) {
+  hipLaunchKernelGGL(kc.fn, dim3(blocks), dim3(kc.waves * 64), 0, st, P);
+  return hipGetLastError();
+}
+hipError_t launch_init_queue(hipStream_t st, unsigned long long* seq, unsigned long long cap) {
+  hipLaunchKernelGGL(mbe::init_queue, dim3(256), dim3(256), 0, st, seq, cap);
+  return hipGetLastError();
+}
+hipError_t launch_reset_ctx(hipStream_t st, const mbe::Params& P, unsigned long long n) {
+  hipLaunchKernelGGL(mbe::reset_ctx, dim3(64), dim3(256), 0, st, P, n);
+  return hipGetLastError();
+}
+hipError_t launch_fill(hipStream_t st, unsigned long long* p, unsigned long long n,
+                       unsigned long long v) {
+  if (n == 0) return hipSuccess;
+  const unsigned long long blocks = std::min<unsigned long long>((n + 255) / 256, 1024);
+  hipLaunchKernelGGL(mbe::fill_u64, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+hipError_t launch_ctl_start(hipStream_t st, unsigned long long* ctl, unsigned long long n_roots) {
+  hipLaunchKernelGGL(mbe::ctl_start, dim3(1), dim3(256), 0, st, ctl, kCtlWords,
+                     (unsigned long long)C_WORK_LEFT * kCtlStride, n_roots);
+  return hipGetLastError();
+}
+hipError_t launch_ctl_wide(hipStream_t st, unsigned long long* ctl, unsigned long long cap) {
+  hipLaunchKernelGGL(mbe::ctl_wide, dim3(1), dim3(256), 0, st, ctl, kCtlWords,
+                     (unsigned long long)C_WORK_LEFT * kCtlStride,
+                     (unsigned long long)C_DEFERRED * kCtlStride,
+                     (unsigned long long)C_WIDE_N * kCtlStride,
+                     (unsigned long long)C_LIST_CURSOR * kCtlStride, cap);
+  return hipGetLastError();
 }
 
 int Engine::ensure_device(int dev) {
@@ -306,7 +378,6 @@ struct gmbe_finder {
   int64_t root_begin = 0, root_end = -1;  // processing-order rank range, -1 = all
   int64_t deadline_ms = 0;                 // 0 = run to completion
   int64_t late_share = 32;                 // 0 = off
-  int64_t occupancy = 2;                   // waves per SIMD the kernels are built for
   int64_t ctx_slots = 4;                   // shared-context slots per wave
   int64_t profile = 0, group = 1, share = 1, ctx_kb = 4096, share_work = 4096, share_queue = -1;
   int64_t merge_min = 256, merge_branch = 16;
@@ -328,6 +399,7 @@ struct gmbe_finder {
   uint32_t* list = nullptr;
   uint64_t list_words = 0;
   mbe::Task* deferred = nullptr;
+  mbe::DevGraph* gt = nullptr;  // device graph table (one entry)
   uint64_t deferred_cap = 1ull << 20;
   uint64_t last_deferred = 0;
   hipStream_t stream = nullptr;
@@ -339,12 +411,8 @@ struct gmbe_finder {
 using namespace gmbe;
 
 static int finder_alloc(gmbe_finder* f) {
-  KernelChoice kc = pick_kernel((int)f->lds_kb, (int)f->occupancy);
-  int per_sm = 0;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_sm, (const void*)kc.fn, kc.waves * 64,
-                                                       0));
-  if (per_sm < 1) per_sm = 1;
-  int blocks = f->blocks > 0 ? (int)f->blocks : per_sm * f->eng.cu_count;
+  KernelChoice kc = pick_kernel((int)f->lds_kb, 4, false);
+  int blocks = f->blocks > 0 ? (int)f->blocks : resident_blocks(kc, f->eng.cu_count);
   int nwaves = blocks * kc.waves;
   mbe::Scratch lay = scratch_layout(f->dg.nV, f->dg.dmaxV, (uint64_t)f->arena_kb * 1024ull,
                                     (uint64_t)f->ctx_kb * 1024ull, (uint32_t)f->ctx_slots);
@@ -419,6 +487,8 @@ int gmbe_finder_create(const gmbe_graph* g, int sel, int bound_height, int bound
     rc = set_error(GMBE_ERR_NOMEM, "control block");
   if (rc == GMBE_OK && hipMalloc(&f->acc, mbe::A_COUNT * sizeof(unsigned long long)) != hipSuccess)
     rc = set_error(GMBE_ERR_NOMEM, "result block");
+  if (rc == GMBE_OK && hipMalloc(&f->gt, sizeof(mbe::DevGraph)) != hipSuccess)
+    rc = set_error(GMBE_ERR_NOMEM, "graph table");
   if (rc) {
     gmbe_finder_destroy(f);
     return rc;
@@ -495,7 +565,15 @@ int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res) {
                      (unsigned long long)f->qcap);
   HIP_TRY(hipGetLastError());
   mbe::Params P{};
-  P.g = f->dg.view();
+  {
+    // the graph table of a finder: one entry
+    mbe::DevGraph e = f->dg.view();
+    e.bound_height = f->bound_height;
+    e.bound_size = f->bound_size;
+    HIP_TRY(hipMemcpyAsync(f->gt, &e, sizeof(e), hipMemcpyHostToDevice, f->stream));
+  }
+  P.gt = f->gt;
+  P.per_graph = nullptr;
   P.s.root_next = f->ctl + C_ROOT_NEXT * kCtlStride;
   P.s.work_left = f->ctl + C_WORK_LEFT * kCtlStride;
   P.s.qctl = f->ctl + C_QCTL * kCtlStride;
@@ -515,8 +593,6 @@ int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res) {
   P.acc = f->acc;
   P.list = f->list;
   P.list_cap = f->list_words;
-  P.bound_height = f->bound_height;
-  P.bound_size = f->bound_size;
   P.prune = (int)f->prune;
   P.max_prefix = (int)f->max_prefix;
   P.split = f->sel == 2;
@@ -529,7 +605,7 @@ int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res) {
   P.late_share = f->share && f->sel == 2 ? (int)f->late_share : 0;  // a form of splitting
   P.share_work = (int)std::min<int64_t>(f->share_work, INT32_MAX);
   P.share_queue = f->share_queue >= 0 ? (int)f->share_queue : f->nwaves;
-  KernelChoice kc = pick_kernel((int)f->lds_kb, (int)f->occupancy);
+  KernelChoice kc = pick_kernel((int)f->lds_kb, 4, false);
   hipLaunchKernelGGL(mbe::reset_ctx, dim3(64), dim3(256), 0, f->stream, P,
                      (unsigned long long)f->nwaves * (unsigned long long)f->ctx_slots);
   HIP_TRY(hipGetLastError());
@@ -561,7 +637,7 @@ int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res) {
     P.s.n_roots = f->last_deferred;
     P.s.deferred = nullptr;
     P.s.deferred_cap = 0;
-    KernelChoice wk = pick_kernel(8, 1, true);
+    KernelChoice wk = pick_kernel(8, 4, true);
     hipLaunchKernelGGL(wk.fn, dim3(f->nblocks), dim3(wk.waves * 64), 0, f->stream, P);
     HIP_TRY(hipGetLastError());
   }
@@ -660,6 +736,7 @@ void gmbe_finder_destroy(gmbe_finder* f) {
   (void)hipFree(f->qseq);
   (void)hipFree(f->ctl);
   (void)hipFree(f->acc);
+  (void)hipFree(f->gt);
   (void)hipFree(f->list);
   (void)hipFree(f->deferred);
   if (f->ev0) (void)hipEventDestroy(f->ev0);

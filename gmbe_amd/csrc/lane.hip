@@ -1,0 +1,317 @@
+// lane.hip — device buffers and launch sequence of one multi-graph run.
+#include "lane.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace gmbe {
+
+#define LANE_TRY(expr)                                                                       \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return set_error(GMBE_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                       __FILE__, __LINE__);                                                  \
+  } while (0)
+
+using mbe::kCtlStride;
+using mbe::kPerGraph;
+
+// ---------------------------------------------------------------------------
+// LanePack
+
+uint32_t LanePack::add(const gmbe_graph& g, int bh, int bs) {
+  const uint32_t id = (uint32_t)table.size();
+  mbe::DevGraph d{};
+  // offsets into the concatenated arrays, turned into pointers by Lane::upload
+  d.v_off = (const int64_t*)(uintptr_t)voff.size();
+  d.v_adj = (const uint32_t*)(uintptr_t)vadj.size();
+  d.u_off = (const int64_t*)(uintptr_t)uoff.size();
+  d.u_adj = (const uint32_t*)(uintptr_t)uadj.size();
+  d.kU = (const uint64_t*)(uintptr_t)kU.size();
+  d.kV = (const uint64_t*)(uintptr_t)kV.size();
+  d.nU = g.nU;
+  d.nV = g.nV;
+  uint32_t dmax = 0;
+  for (uint32_t v = 0; v < g.nV; ++v)
+    dmax = std::max<uint32_t>(dmax, (uint32_t)(g.v_off[v + 1] - g.v_off[v]));
+  d.dmaxV = dmax;
+  d.swapped = (uint32_t)g.swapped;
+  d.bound_height = bh;
+  d.bound_size = bs;
+  voff.insert(voff.end(), g.v_off.begin(), g.v_off.end());
+  uoff.insert(uoff.end(), g.u_off.begin(), g.u_off.end());
+  vadj.insert(vadj.end(), g.v_adj.begin(), g.v_adj.end());
+  uadj.insert(uadj.end(), g.u_adj.begin(), g.u_adj.end());
+  std::vector<uint64_t> ku, kv;
+  hash_keys(g, ku, kv);
+  kU.insert(kU.end(), ku.begin(), ku.end());
+  kV.insert(kV.end(), kv.begin(), kv.end());
+  table.push_back(d);
+  max_nV = std::max(max_nV, g.nV);
+  max_dmaxV = std::max(max_dmaxV, dmax);
+  return id;
+}
+
+void LanePack::finish() {
+  const uint32_t n = (uint32_t)table.size();
+  std::vector<double> cost(n, 0.0);
+  for (uint32_t i = 0; i < n; ++i) {
+    const mbe::DevGraph& d = table[i];
+    const int64_t* uo = uoff.data() + (uintptr_t)d.u_off;
+    double c = 0;
+    for (uint32_t u = 0; u < d.nU; ++u) {
+      const double du = (double)(uo[u + 1] - uo[u]);
+      c += du * du;
+    }
+    cost[i] = c;
+  }
+  order.resize(n);
+  for (uint32_t i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+  root_base.assign(n + 1, 0ull);
+  for (uint32_t j = 0; j < n; ++j) root_base[j + 1] = root_base[j] + table[order[j]].nV;
+}
+
+void LanePack::clear() {
+  voff.clear();
+  uoff.clear();
+  vadj.clear();
+  uadj.clear();
+  kU.clear();
+  kV.clear();
+  table.clear();
+  order.clear();
+  root_base.clear();
+  max_nV = max_dmaxV = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Lane
+
+int Lane::alloc(Engine* e, const LaneCaps& c, const LaneKnobs& k) {
+  eng = e;
+  caps = c;
+  knobs = k;
+  LANE_TRY(hipSetDevice(eng->device));
+  LANE_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  LANE_TRY(hipEventCreate(&ev0));
+  LANE_TRY(hipEventCreate(&ev1));
+  const uint64_t ng = std::max<uint32_t>(c.max_graphs, 1);
+  LANE_TRY(hipMalloc(&voff, std::max<uint64_t>(c.max_voff, 1) * 8));
+  LANE_TRY(hipMalloc(&uoff, std::max<uint64_t>(c.max_uoff, 1) * 8));
+  LANE_TRY(hipMalloc(&vadj, std::max<uint64_t>(c.max_edges, 1) * 4));
+  LANE_TRY(hipMalloc(&uadj, std::max<uint64_t>(c.max_edges, 1) * 4));
+  LANE_TRY(hipMalloc(&kU, std::max<uint64_t>(c.max_uoff, 1) * 8));
+  LANE_TRY(hipMalloc(&kV, std::max<uint64_t>(c.max_voff, 1) * 8));
+  LANE_TRY(hipMalloc(&gt, ng * sizeof(mbe::DevGraph)));
+  LANE_TRY(hipMalloc(&root_base, (ng + 1) * 8));
+  LANE_TRY(hipMalloc(&order, ng * 4));
+  LANE_TRY(hipMalloc(&per_graph, ng * kPerGraph * 8));
+  LANE_TRY(hipMalloc(&ctl, kCtlWords * 8));
+  LANE_TRY(hipMalloc(&acc, mbe::A_COUNT * 8));
+  LANE_TRY(hipMalloc(&deferred, knobs.deferred_cap * sizeof(mbe::Task)));
+  stage_bytes = (c.max_voff + c.max_uoff) * 16 + c.max_edges * 8 +
+                ng * (sizeof(mbe::DevGraph) + 8 + 4) + 8;
+  LANE_TRY(hipHostMalloc((void**)&stage, stage_bytes, hipHostMallocDefault));
+  LANE_TRY(hipHostMalloc((void**)&out_host, (ng * kPerGraph + mbe::A_COUNT) * 8,
+                         hipHostMallocDefault));
+  return alloc_run();
+}
+
+int Lane::alloc_run() {
+  kc = pick_kernel(knobs.lds_kb, knobs.waves, false);
+  wk = pick_kernel(8, 4, true);
+  if (!kc.fn)
+    return set_error(GMBE_ERR_ARG, "no kernel with %d waves per block and lds_kb=%d",
+                     knobs.waves, knobs.lds_kb);
+  const int blocks = knobs.blocks > 0 ? knobs.blocks : resident_blocks(kc, eng->cu_count);
+  // the wide pass runs on at most as many waves as the main pass
+  wblocks = std::max(1, std::min(resident_blocks(wk, eng->cu_count), blocks * kc.waves / wk.waves));
+  const int nw = blocks * kc.waves;
+  mbe::Scratch lay = scratch_layout(std::max<uint32_t>(caps.max_nV, 1),
+                                    std::max<uint32_t>(caps.max_dmaxV, 1),
+                                    (uint64_t)knobs.arena_kb * 1024ull,
+                                    (uint64_t)knobs.ctx_kb * 1024ull, (uint32_t)knobs.ctx_slots);
+  const uint64_t bytes = lay.slab * (uint64_t)nw;
+  if (bytes != scratch_bytes) {
+    if (scratch) LANE_TRY(hipFree(scratch));
+    scratch = nullptr;
+    scratch_bytes = 0;
+    hipError_t e = hipMalloc(&scratch, bytes);
+    if (e != hipSuccess)
+      return set_error(GMBE_ERR_NOMEM, "lane scratch of %llu bytes (%d waves): %s",
+                       (unsigned long long)bytes, nw, hipGetErrorString(e));
+    scratch_bytes = bytes;
+    // slabs start zeroed (the candidate maps rely on it); a kernel, not a
+    // memset, so that nothing here differs between eager and captured use
+    LANE_TRY(launch_fill(st, (unsigned long long*)scratch, bytes / 8, 0ull));
+  }
+  lay.base = scratch;
+  layout = lay;
+  nwaves = nw;
+  nblocks = blocks;
+  uint64_t qc = (1ull << knobs.queue_log2) / mbe::kShards;
+  if (qc < 16) qc = 16;
+  if (qc != qcap) {
+    if (q) LANE_TRY(hipFree(q));
+    if (qseq) LANE_TRY(hipFree(qseq));
+    q = nullptr;
+    qseq = nullptr;
+    LANE_TRY(hipMalloc(&q, qc * mbe::kShards * mbe::kEntryWords * 4));
+    LANE_TRY(hipMalloc(&qseq, qc * mbe::kShards * 8));
+    qcap = qc;
+  }
+  LANE_TRY(hipStreamSynchronize(st));
+  return GMBE_OK;
+}
+
+void Lane::release() {
+  if (eng && eng->device >= 0) (void)hipSetDevice(eng->device);
+  if (st) (void)hipStreamSynchronize(st);
+  void* dev[] = {voff, uoff, vadj, uadj, kU, kV, gt, root_base, order, per_graph,
+                 scratch, q, qseq, ctl, acc, deferred};
+  for (void* p : dev)
+    if (p) (void)hipFree(p);
+  if (stage) (void)hipHostFree(stage);
+  if (out_host) (void)hipHostFree(out_host);
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  if (st) (void)hipStreamDestroy(st);
+  *this = Lane();
+}
+
+int Lane::upload(const LanePack& p) {
+  const uint32_t n = (uint32_t)p.table.size();
+  if (n > caps.max_graphs || p.voff.size() > caps.max_voff || p.uoff.size() > caps.max_uoff ||
+      p.vadj.size() > caps.max_edges || p.max_nV > caps.max_nV || p.max_dmaxV > caps.max_dmaxV)
+    return set_error(GMBE_ERR_CAPACITY, "graph set exceeds the lane's capacity");
+  // stage: voff | uoff | kU | kV | vadj | uadj | table | root_base | order
+  char* s = stage;
+  auto put = [&](const void* src, uint64_t bytes, void* dst) -> hipError_t {
+    if (bytes == 0) return hipSuccess;
+    memcpy(s, src, bytes);
+    hipError_t e = hipMemcpyAsync(dst, s, bytes, hipMemcpyHostToDevice, st);
+    s += (bytes + 15) / 16 * 16;
+    return e;
+  };
+  std::vector<mbe::DevGraph> tab(p.table);
+  for (mbe::DevGraph& d : tab) {
+    d.v_off = voff + (uintptr_t)d.v_off;
+    d.v_adj = vadj + (uintptr_t)d.v_adj;
+    d.u_off = uoff + (uintptr_t)d.u_off;
+    d.u_adj = uadj + (uintptr_t)d.u_adj;
+    d.kU = kU + (uintptr_t)d.kU;
+    d.kV = kV + (uintptr_t)d.kV;
+  }
+  LANE_TRY(put(p.voff.data(), p.voff.size() * 8, voff));
+  LANE_TRY(put(p.uoff.data(), p.uoff.size() * 8, uoff));
+  LANE_TRY(put(p.kU.data(), p.kU.size() * 8, kU));
+  LANE_TRY(put(p.kV.data(), p.kV.size() * 8, kV));
+  LANE_TRY(put(p.vadj.data(), p.vadj.size() * 4, vadj));
+  LANE_TRY(put(p.uadj.data(), p.uadj.size() * 4, uadj));
+  LANE_TRY(put(tab.data(), tab.size() * sizeof(mbe::DevGraph), gt));
+  LANE_TRY(put(p.root_base.data(), p.root_base.size() * 8, root_base));
+  LANE_TRY(put(p.order.data(), p.order.size() * 4, order));
+  n_graphs = n;
+  n_roots = p.total_roots();
+  return GMBE_OK;
+}
+
+mbe::Params Lane::params(bool wide) const {
+  mbe::Params P{};
+  P.gt = gt;
+  P.per_graph = per_graph;
+  P.s.root_next = ctl + C_ROOT_NEXT * kCtlStride;
+  P.s.work_left = ctl + C_WORK_LEFT * kCtlStride;
+  P.s.qctl = ctl + C_QCTL * kCtlStride;
+  P.s.queued = ctl + C_QUEUED * kCtlStride;
+  P.list_cursor = ctl + C_LIST_CURSOR * kCtlStride;
+  P.s.n_deferred = ctl + C_DEFERRED * kCtlStride;
+  P.s.q = q;
+  P.s.qseq = qseq;
+  P.s.qcap = qcap;
+  P.s.stride = 1;
+  P.s.phase = 0;
+  P.s.n_graphs = n_graphs;
+  if (!wide) {
+    P.s.n_roots = n_roots;
+    P.s.root_base = root_base;
+    P.s.order = order;
+    P.s.deferred = deferred;
+    P.s.deferred_cap = knobs.deferred_cap;
+  } else {
+    P.s.n_roots_dev = ctl + C_WIDE_N * kCtlStride;
+    P.s.root_tasks = deferred;
+  }
+  P.w = layout;
+  P.acc = acc;
+  P.list = nullptr;
+  P.list_cap = 0;
+  P.prune = knobs.prune;
+  P.max_prefix = knobs.max_prefix;
+  P.split = knobs.split;
+  P.profile = 0;
+  P.group = knobs.group;
+  P.share = knobs.share && knobs.split;
+  P.merge_min = (int)knobs.merge_min;
+  P.merge_branch = (int)knobs.merge_branch;
+  P.share_work = (int)std::min<int64_t>(knobs.share_work, INT32_MAX);
+  P.share_queue = knobs.share_queue >= 0 ? (int)knobs.share_queue : nwaves;
+  P.late_share = P.share ? (int)knobs.late_share : 0;
+  P.budget = (uint64_t)knobs.budget_ms * (uint64_t)eng->wall_khz;
+  return P;
+}
+
+int Lane::record_run() {
+  const mbe::Params P = params(false), Pw = params(true);
+  const uint64_t ng = std::max<uint32_t>(n_graphs, 1);
+  LANE_TRY(launch_ctl_start(st, ctl, n_roots));
+  LANE_TRY(launch_fill(st, acc, mbe::A_COUNT, 0ull));
+  LANE_TRY(launch_fill(st, per_graph, ng * kPerGraph, 0ull));
+  LANE_TRY(launch_init_queue(st, qseq, qcap));
+  LANE_TRY(launch_reset_ctx(st, P, (unsigned long long)nwaves * layout.ctx_slots));
+  if (n_roots > 0) LANE_TRY(launch_persistent(kc, nblocks, st, P));
+  // wide pass over the deferred tasks; its grid exits at once when there are none
+  LANE_TRY(launch_ctl_wide(st, ctl, knobs.deferred_cap));
+  LANE_TRY(launch_init_queue(st, qseq, qcap));
+  if (n_roots > 0) LANE_TRY(launch_persistent(wk, wblocks, st, Pw));
+  LANE_TRY(hipMemcpyAsync(out_host, per_graph, ng * kPerGraph * 8, hipMemcpyDeviceToHost, st));
+  LANE_TRY(hipMemcpyAsync(out_host + ng * kPerGraph, acc, mbe::A_COUNT * 8,
+                          hipMemcpyDeviceToHost, st));
+  return GMBE_OK;
+}
+
+void Lane::result(uint32_t i, gmbe_result* r) const {
+  memset(r, 0, sizeof(*r));
+  const unsigned long long* a = out_host + (uint64_t)i * kPerGraph;
+  r->maximal = a[mbe::G_MAXIMAL];
+  r->hash_sum = a[mbe::G_HSUM];
+  r->hash_xor = a[mbe::G_HXOR];
+  r->non_maximal = a[mbe::G_NONMAX];
+  r->nodes = a[mbe::G_NODES];
+  r->overflow = a[mbe::G_OVERFLOW];
+  r->alg_bytes = a[mbe::G_BALG];
+  r->tasks = a[mbe::G_TASKS];
+}
+
+void Lane::totals(gmbe_result* r) const {
+  memset(r, 0, sizeof(*r));
+  for (uint32_t i = 0; i < n_graphs; ++i) {
+    gmbe_result g;
+    result(i, &g);
+    r->maximal += g.maximal;
+    r->hash_sum += g.hash_sum;
+    r->hash_xor ^= g.hash_xor;
+    r->non_maximal += g.non_maximal;
+    r->nodes += g.nodes;
+    r->overflow += g.overflow;
+    r->alg_bytes += g.alg_bytes;
+    r->tasks += g.tasks;
+  }
+  const unsigned long long* a = out_host + (uint64_t)std::max<uint32_t>(n_graphs, 1) * kPerGraph;
+  r->splits = a[mbe::A_SPLITS];
+}
+
+}  // namespace gmbe

@@ -49,8 +49,10 @@ constexpr int kMainW = 4;                  // widest bitset of the main kernel
 
 struct Task {
   uint32_t v[kMaxPrefix];
+  uint32_t gid;  // graph of the task (index into the graph table)
 };
 
+// One entry of the device graph table (a finder has one, a batch many).
 struct DevGraph {
   const int64_t* v_off;
   const uint32_t* v_adj;
@@ -59,7 +61,19 @@ struct DevGraph {
   const uint64_t* kU;
   const uint64_t* kV;
   uint32_t nU, nV, dmaxV, swapped;
+  int32_t bound_height, bound_size;  // split thresholds for this graph
+  uint32_t pad[2];
 };
+
+// Header word of a queue entry of split children: prefix length k (<= 8),
+// child count (<= kChunk) and graph id.  Bit 31 is clear (item entries set it).
+__host__ __device__ __forceinline__ uint32_t child_hdr(uint32_t k, uint32_t cnt, uint32_t gid) {
+  return k | (cnt << 4) | (gid << 9);
+}
+__device__ __forceinline__ uint32_t child_hdr_k(uint32_t h) { return h & 0xfu; }
+__device__ __forceinline__ uint32_t child_hdr_cnt(uint32_t h) { return (h >> 4) & 0x1fu; }
+__device__ __forceinline__ uint32_t child_hdr_gid(uint32_t h) { return (h >> 9) & 0x3fffffu; }
+constexpr uint32_t kMaxGraphs = 1u << 22;
 
 // Counter slots of the result block (u64 each).
 enum Acc : int {
@@ -81,6 +95,10 @@ struct Sched {
   unsigned long long* queued;     // entries pushed and not yet popped (a hint)
   unsigned long long qcap;        // slots per shard, power of two
   unsigned long long n_roots;     // top-level vertices of this run (sample)
+  const unsigned long long* n_roots_dev;  // if set, the count is read here instead
+  const unsigned long long* root_base;    // batch: first root of the j-th graph in order
+  const uint32_t* order;                  // batch: graph id of the j-th graph in order
+  uint32_t n_graphs;
   uint32_t stride, phase;         // root r -> vertex nV-1-(r*stride+phase) (phase = first rank)
   const Task* root_tasks;         // if set, root r is root_tasks[r] instead
   Task* deferred;                 // tasks too wide for this kernel's bitsets
@@ -100,14 +118,15 @@ struct Scratch {
 };
 
 struct Params {
-  DevGraph g;
+  const DevGraph* gt;        // graph table (device memory)
+  unsigned long long* per_graph;  // batch: kPerGraph counters per graph, or null
   Sched s;
   Scratch w;
   unsigned long long* acc;   // A_COUNT counters
   uint32_t* list;            // list-mode output (internal ids) or null
   unsigned long long* list_cursor;
   unsigned long long list_cap;
-  int bound_height, bound_size, prune, max_prefix, split;
+  int prune, max_prefix, split;
   int profile;               // accumulate per-phase cycle counts
   int group;                 // group twin candidates (one-word bitsets)
   int share;                 // split large tasks through shared contexts
@@ -132,6 +151,28 @@ struct Counters {
   uint64_t abandoned = 0;              // subtrees left unfinished at the deadline
   uint64_t late = 0;                   // frames handed over by late sharing
 };
+
+// Per-graph result counters of a batch (u64 each, kPerGraph per graph).
+enum PerGraph : int {
+  G_MAXIMAL = 0, G_HSUM, G_HXOR, G_NONMAX, G_NODES, G_OVERFLOW, G_BALG, G_TASKS, kPerGraph
+};
+
+// Move this wave's result counters into graph gid's slots (batch only).
+__device__ __forceinline__ void flush_graph(const Params& P, Counters& c, uint32_t gid) {
+  if (lane_id() == 0) {
+    unsigned long long* a = P.per_graph + (uint64_t)gid * kPerGraph;
+    const uint64_t add[kPerGraph] = {c.maximal, c.hsum, 0, c.nonmax, c.nodes, c.overflow, c.balg,
+                                     c.tasks};
+    for (int i = 0; i < kPerGraph; ++i)
+      if (i != G_HXOR && add[i] != 0)
+        __hip_atomic_fetch_add(a + i, (unsigned long long)add[i], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    if (c.hxor != 0)
+      __hip_atomic_fetch_xor(a + G_HXOR, (unsigned long long)c.hxor, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+  }
+  c.maximal = c.hsum = c.hxor = c.nonmax = c.nodes = c.overflow = c.balg = c.tasks = 0;
+}
 
 // True once this wave's deadline has passed (never without a budget).  The
 // constant-rate wall clock is read per wave; no memory is touched.
@@ -377,7 +418,7 @@ __device__ bool queue_push_children(const Sched& s, const Task& prefix, int k,
       const uint64_t slot = slot_of(s, sh, start + e);
       uint32_t cnt = n - e * kChunk < (uint32_t)kChunk ? n - e * kChunk : (uint32_t)kChunk;
       uint32_t val;
-      if (w == 0) val = (uint32_t)k | (cnt << 8);
+      if (w == 0) val = child_hdr((uint32_t)k, cnt, prefix.gid);
       else if (w <= 7) val = w - 1 < (uint32_t)k ? prefix.v[w - 1] : kNone;
       else val = (w - 8) < cnt ? cand[e * kChunk + (w - 8)] : kNone;
       st_dev(s.q + slot * kEntryWords + w, val);
@@ -392,6 +433,12 @@ __device__ bool queue_push_children(const Sched& s, const Task& prefix, int k,
   return true;
 }
 
+// Top-level vertices of this launch (read from memory when a previous launch
+// decided them, as for the wide pass of a captured batch).
+__device__ __forceinline__ unsigned long long roots_total(const Sched& s) {
+  return s.n_roots_dev ? uni64(ld_dev(s.n_roots_dev)) : s.n_roots;
+}
+
 // Next unit of work: a queue entry (split children) first, then a top-level
 // vertex in descending id (src/IterFinderGpu.cu:992-1018).  Returns 0 once
 // every unit has finished (the gc/lc/tc protocol of
@@ -401,15 +448,16 @@ __device__ int next_work(const Params& P, uint32_t& word, Task& t, int& k, uint3
   const Sched& s = P.s;
   uint32_t lane = lane_id();
   uint32_t nap = 1;
+  const unsigned long long n_roots = roots_total(s);
   for (;;) {
     if (past_deadline(P)) return 0;  // leave what is queued: the run is cut short
     if (queue_pop(s, word, home)) return 1;
     unsigned long long r = ~0ull;
-    if (lane == 0 && ld_dev(s.root_next) < s.n_roots) {
+    if (lane == 0 && ld_dev(s.root_next) < n_roots) {
       r = __hip_atomic_fetch_add(s.root_next, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     r = uni64(r);
-    if (r != ~0ull && r < s.n_roots) {
+    if (r != ~0ull && r < n_roots) {
       if (s.root_tasks) {
         uint32_t w = lane < (uint32_t)kMaxPrefix ? s.root_tasks[r].v[lane] : kNone;
         k = 0;
@@ -417,11 +465,25 @@ __device__ int next_work(const Params& P, uint32_t& word, Task& t, int& k, uint3
           t.v[i] = read_lane(w, i);
           k += t.v[i] != kNone;
         }
+        t.gid = uni(s.root_tasks[r].gid);
         return 2;
       }
+      uint32_t gid = 0;
       uint64_t rank = r * s.stride + s.phase;
+      if (s.root_base) {
+        // batch: the graph whose root range holds r (graphs in processing order)
+        uint32_t lo = 0, hi = s.n_graphs;  // root_base[lo] <= r < root_base[hi]
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (s.root_base[mid] <= r) lo = mid; else hi = mid;
+        }
+        lo = uni(lo);
+        gid = uni(s.order[lo]);
+        rank = r - s.root_base[lo];
+      }
       for (int i = 0; i < kMaxPrefix; ++i) t.v[i] = kNone;
-      t.v[0] = P.g.nV - 1 - (uint32_t)rank;
+      t.v[0] = P.gt[gid].nV - 1 - (uint32_t)rank;
+      t.gid = gid;
       k = 1;
       return 2;
     }
@@ -460,7 +522,12 @@ struct WaveMem {
   uint32_t lds_bytes;
   uint32_t* s_off; // 64 + 1 u32 in LDS
   int64_t* s_start;// 64 i64 in LDS
+  const DevGraph* g;  // graph of the current unit of work (set per unit)
 };
+
+__device__ __forceinline__ uint32_t graph_id(const Params& P, const WaveMem& m) {
+  return (uint32_t)(m.g - P.gt);
+}
 
 __device__ __forceinline__ WaveMem wave_mem(const Params& P, uint64_t gw, char* lds,
                                             uint32_t lds_bytes, uint32_t* s_off,
@@ -487,6 +554,7 @@ __device__ __forceinline__ WaveMem wave_mem(const Params& P, uint64_t gw, char* 
   m.lds_bytes = lds_bytes;
   m.s_off = s_off;
   m.s_start = s_start;
+  m.g = P.gt;
   return m;
 }
 
@@ -717,7 +785,8 @@ struct CtxHdr {
   unsigned long long refs;  // items not yet finished
   uint32_t nL, nC, len, W;  // task |L| and |C|, list length, bitset words
   uint32_t nbits;           // |L| at the shared node
-  uint32_t pad[9];
+  uint32_t gid;             // graph of the shared node
+  uint32_t pad[8];
 };
 
 // Context id = wave * ctx_slots + slot; a wave publishes only into its own
@@ -763,8 +832,8 @@ constexpr int kItemsPerEntry = 15;  // (position, candidate) pairs after hdr, ow
 // (nothing published) when the queue lacks room or fewer than min_items
 // entries are left to push.
 template <int W, bool KL_R, bool C_R>
-__device__ bool publish_list(const Params& P, uint64_t gw, uint32_t ctx, const Rec<W>* root,
-                             uint32_t len,
+__device__ bool publish_list(const Params& P, uint64_t gw, uint32_t ctx, uint32_t gid,
+                             const Rec<W>* root, uint32_t len,
                              uint32_t nL, uint32_t nC, uint32_t nbits, uint64_t sumR0,
                              const TaskArrays& src, uint32_t min_rep = 0,
                              uint32_t min_items = 1) {
@@ -806,6 +875,7 @@ __device__ bool publish_list(const Params& P, uint64_t gw, uint32_t ctx, const R
     st_dev(&h->nC, nC);
     st_dev(&h->len, len);
     st_dev(&h->W, (uint32_t)W);
+    st_dev(&h->gid, gid);
     st_dev(&h->refs, (unsigned long long)n_items);
   }
   wave_drain();
@@ -973,7 +1043,7 @@ __device__ __forceinline__ uint32_t share_wanted(const Params& P, uint64_t gw, u
   const uint64_t bytes = sizeof(CtxHdr) + (uint64_t)len * (16 + 8 * W) + (uint64_t)nC * 4 +
                          (uint64_t)nL * 8;
   if (bytes > P.w.ctx_bytes) return kNone;
-  if (uni64(ld_dev(P.s.root_next)) < P.s.n_roots) return kNone;
+  if (uni64(ld_dev(P.s.root_next)) < roots_total(P.s)) return kNone;
   const long long depth = (long long)uni64(ld_dev(P.s.queued));
   if (depth >= (long long)P.share_queue) return kNone;
   return free_ctx(P, gw);
@@ -1042,7 +1112,7 @@ __device__ bool child_node(const Params& P, const WaveMem& m, uint64_t gw, Count
                                 : kNone;
         if (cx != kNone) {
           uint64_t tq = P.profile ? cycles() : 0;
-          shared = publish_list<1, false, C_R>(P, gw, cx, dst, dlen, nb, nC, nb, sumR, tc);
+          shared = publish_list<1, false, C_R>(P, gw, cx, graph_id(P, m), dst, dlen, nb, nC, nb, sumR, tc);
           if (P.profile) ctr.c_publish += cycles() - tq;
         }
         if (shared) {
@@ -1082,7 +1152,7 @@ __device__ bool child_node(const Params& P, const WaveMem& m, uint64_t gw, Count
                           : kNone;
   if (cx != kNone) {
     uint64_t tq = P.profile ? cycles() : 0;
-    bool ok = publish_list<W, KL_R, C_R>(P, gw, cx, crec, clen, nL, nC, nb, sumR, ta);
+    bool ok = publish_list<W, KL_R, C_R>(P, gw, cx, graph_id(P, m), crec, clen, nL, nC, nb, sumR, ta);
     if (P.profile) ctr.c_publish += cycles() - tq;
     if (ok) {
       ctr.splits++;
@@ -1100,12 +1170,13 @@ __device__ bool child_node(const Params& P, const WaveMem& m, uint64_t gw, Count
 // unpushable in place), so publishing that suffix and closing the frame keeps
 // the traversal's result.  Returns true when the current frame was closed.
 template <int W, bool KL_R, bool C_R>
-__device__ bool late_share(const Params& P, uint64_t gw, uint32_t nL, uint32_t nC,
+__device__ bool late_share(const Params& P, const WaveMem& m, uint64_t gw, uint32_t nL,
+                           uint32_t nC,
                            const TaskArrays& ta, Frame& F, Frame* fr, uint32_t depth,
                            Counters& ctr) {
   // cheap gate first: the top-level vertices are all claimed and the queue is
   // short; then pick the shallowest frame with pushes left
-  if (uni64(ld_dev(P.s.root_next)) < P.s.n_roots) return false;
+  if (uni64(ld_dev(P.s.root_next)) < roots_total(P.s)) return false;
   if ((long long)uni64(ld_dev(P.s.queued)) >= (long long)P.share_queue) return false;
   uint32_t at = kNone;
   Frame S;
@@ -1130,7 +1201,7 @@ __device__ bool late_share(const Params& P, uint64_t gw, uint32_t nL, uint32_t n
   const uint32_t cx = share_wanted(P, gw, nL, nC, S.len, W);
   if (cx == kNone) return false;
   uint64_t tq = P.profile ? cycles() : 0;
-  const bool ok = publish_list<W, KL_R, C_R>(P, gw, cx, (const Rec<W>*)S.rec, S.len, nL, nC,
+  const bool ok = publish_list<W, KL_R, C_R>(P, gw, cx, graph_id(P, m), (const Rec<W>*)S.rec, S.len, nL, nC,
                                              S.nbits, S.sumR, ta, S.curidx, 2);
   if (P.profile) ctr.c_publish += cycles() - tq;
   if (!ok) return false;
@@ -1155,6 +1226,8 @@ __device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC,
   typedef Rec<W> R_t;
   const uint32_t lane = lane_id();
   const bool prune = P.prune != 0;
+  const int64_t* voff = m.g->v_off;
+  const uint32_t swapped = m.g->swapped;
   uint32_t depth = 0, tick = 0;
   for (;;) {
     ++tick;
@@ -1180,14 +1253,14 @@ __device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC,
       continue;
     }
     if (P.late_share && !P.list && (tick & 7u) == 0 &&
-        late_share<W, KL_R, C_R>(P, gw, nL, nC, ta, F, fr, depth, ctr))
+        late_share<W, KL_R, C_R>(P, m, gw, nL, nC, ta, F, fr, depth, ctr))
       continue;  // the current frame was handed over: F.cur is now kNone
     R_t* recs = (R_t*)F.rec;
     const uint32_t xpos = F.cur, xidx = F.curidx;
     {
       // B(n) = 4 (deg(x) + |L_parent| + |C|)
       uint32_t xc = rd<C_R>(ta.C + xidx);
-      uint64_t dx = (uint64_t)(P.g.v_off[xc + 1] - P.g.v_off[xc]);
+      uint64_t dx = (uint64_t)(voff[xc + 1] - voff[xc]);
       ctr.balg += 4ull * (dx + F.nbits + nC);
     }
     uint64_t ax[W];
@@ -1226,7 +1299,7 @@ __device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC,
     }
     const uint64_t sumL = key_sum_of_bits<W, KL_R>(ax, ta);
     const uint64_t sumR = F.sumR + o.absorbed;
-    add_biclique(ctr, biclique_hash(sumL, sumR, P.g.swapped));
+    add_biclique(ctr, biclique_hash(sumL, sumR, swapped));
     if constexpr (!KL_R && !C_R) {
       if (P.list) {
         // R' = R0 ∪ {C[i] : a_i ⊇ L'} over the (ungrouped) root list
@@ -1318,7 +1391,7 @@ __device__ void run_items_w(const Params& P, const WaveMem& m, CtxHdr* h, uint32
     uint64_t t0 = P.profile ? cycles() : 0;
     {
       uint32_t xc = ld_dev(ta.C + xidx);
-      uint64_t dx = (uint64_t)(P.g.v_off[xc + 1] - P.g.v_off[xc]);
+      uint64_t dx = (uint64_t)(m.g->v_off[xc + 1] - m.g->v_off[xc]);
       ctr.balg += 4ull * (dx + nbits + nC);
     }
     uint64_t ax[W];
@@ -1354,7 +1427,7 @@ __device__ void run_items_w(const Params& P, const WaveMem& m, CtxHdr* h, uint32
     }
     const uint64_t sumL = key_sum_of_bits<W, true>(ax, ta);
     const uint64_t sumR = sumR0 + o.absorbed;
-    add_biclique(ctr, biclique_hash(sumL, sumR, P.g.swapped));
+    add_biclique(ctr, biclique_hash(sumL, sumR, m.g->swapped));
     if (nb > 1 && o.best_child != ~0ull) {
       if (ctier == 0) lds_top = (uint32_t)(ctop + (uint64_t)o.clen * sizeof(R_t));
       else hbm_top = ctop + (uint64_t)o.clen * sizeof(R_t);
@@ -1412,7 +1485,7 @@ __device__ void enumerate_task(const Params& P, const WaveMem& m, const uint32_t
                                uint32_t nC, uint32_t next_id, uint64_t sumR, uint32_t nR,
                                Counters& ctr, uint64_t& t0, uint32_t share_ctx, uint64_t gw) {
   typedef Rec<Wt> R_t;
-  const DevGraph& g = P.g;
+  const DevGraph g = *m.g;
   const uint32_t lane = lane_id();
   int32_t* cmap = m.cmap;
   uint32_t lds_top = 0;
@@ -1454,7 +1527,7 @@ __device__ void enumerate_task(const Params& P, const WaveMem& m, const uint32_t
   wave_mem_wait();
   TaskArrays lta{m.kL, m.C};
   if (share_ctx != kNone &&
-      publish_list<Wt, false, false>(P, gw, share_ctx, (const R_t*)list_addr, len, nL, nC, nL,
+      publish_list<Wt, false, false>(P, gw, share_ctx, graph_id(P, m), (const R_t*)list_addr, len, nL, nC, nL,
                                      sumR, lta)) {
     ctr.splits++;
     phase(ctr, P.profile, 6, t0);
@@ -1543,7 +1616,7 @@ __device__ Prefix prefix_common(const DevGraph& g, const WaveMem& m, const Task&
 template <int WMAX>
 __device__ void run_task(const Params& P, const WaveMem& m, const Task& t, int k,
                          const Prefix& pre, Counters& ctr, uint64_t gw) {
-  const DevGraph& g = P.g;
+  const DevGraph g = *m.g;
   const uint32_t lane = lane_id();
   ctr.tasks++;
   ctr.nodes++;
@@ -1696,8 +1769,8 @@ __device__ void run_task(const Params& P, const WaveMem& m, const Task& t, int k
   }
   const uint32_t W = (nL + 63) / 64;
   // split (src/IterFinderGpu.cu:835-858)
-  bool rule = P.split && ((rest > (uint32_t)P.bound_height || nL > (uint32_t)P.bound_height) &&
-                          (uint64_t)rest * nL > (uint64_t)P.bound_size);
+  bool rule = P.split && ((rest > (uint32_t)g.bound_height || nL > (uint32_t)g.bound_height) &&
+                          (uint64_t)rest * nL > (uint64_t)g.bound_size);
   bool must = W > (uint32_t)kMaxW;  // |L| beyond any bitset: split regardless
   // large task: share its root list with other waves when one of this wave's
   // context slots is free and large enough
@@ -1751,6 +1824,7 @@ __device__ void run_task(const Params& P, const WaveMem& m, const Task& t, int k
       return;
     }
     if (lane < (uint32_t)kMaxPrefix) st_dev(&P.s.deferred[at].v[lane], t.v[lane]);
+    if (lane == (uint32_t)kMaxPrefix) st_dev(&P.s.deferred[at].gid, t.gid);
     return;
   }
   count_root(sumL, sumR);

@@ -72,5 +72,10 @@ def batch_seed(i: int) -> int:
     return (4 << 32) | i
 
 
+def batch_graphs(n: int = 4096, first: int = 0) -> list:
+    """Graphs first..first+n-1 of the batched workload, in working form."""
+    return [small_graph(batch_seed(i)).derive(**RUN_OPTION) for i in range(first, first + n)]
+
+
 def serve_small_seed(j: int) -> int:
     return (5 << 32) | j

@@ -220,13 +220,34 @@ void gmbe_finder_destroy(gmbe_finder* f);
 
 /* ---- batched form ------------------------------------------------------ */
 
-/* Many independent graphs enumerated by one persistent launch (captured in a
- * HIP graph and replayed by gmbe_batch_execute).  No reference counterpart
- * (the reference runs one graph per process). */
+/* Many independent graphs (working form, e.g. from gmbe_graph_derive with
+ * -t 1 -o 1 -f) enumerated by one persistent launch over a concatenated
+ * device CSR with a per-graph table.  The launch sequence is captured once in
+ * a HIP graph and replayed by every gmbe_batch_execute.  Graphs fall in four
+ * size classes by edge count ([0,2k), [2k,8k), [8k,32k), [32k,inf)); each
+ * class has its own split thresholds.  No reference counterpart (the
+ * reference runs one graph per process, src/main.cu:126-147).  The graphs
+ * are borrowed and must outlive the batch. */
 int gmbe_batch_create(const gmbe_graph* const* graphs, int n, int bound_height,
                       int bound_size, gmbe_batch** out);
+/* Knobs: "waves_per_block" (2, 4, 8; built with lds_kb 16, any, 4
+ * respectively: setting it moves lds_kb to a built value), "lds_kb" (LDS
+ * list arena per wave, KB),
+ * "bound_height.<c>", "bound_size.<c>" (class c), "prune", "max_prefix",
+ * "group", "share", "split", "share_work", "merge_min", "merge_branch",
+ * "late_share", "arena_kb", "ctx_kb", "ctx_slots";
+ * "per_graph_launch" = 1 runs the graphs one launch (and one host wait) at a
+ * time with the same kernel, the looped baseline;
+ * "autotune" = 1 measures the kernel variants and then each class's
+ * threshold candidates on this batch and keeps the fastest (synchronous). */
 int gmbe_batch_set(gmbe_batch* b, const char* key, int64_t value);
-/* per_graph may be NULL; otherwise it receives n results (times are zero). */
+/* Read back a knob or a choice of the autotuner: "waves_per_block",
+ * "lds_kb", "n_classes", "bound_height.<c>", "bound_size.<c>", "class.<i>"
+ * (size class of graph i), "kernel_time_us", "wall_time_us" (last execute). */
+int gmbe_batch_get(const gmbe_batch* b, const char* key, int64_t* value);
+/* per_graph may be NULL; otherwise it receives n results in input order
+ * (times zero).  total gets the sums (hash xor combined), the wall time of
+ * the run (graph launch to results on the host) and its event time. */
 int gmbe_batch_execute(gmbe_batch* b, gmbe_result* per_graph,
                        gmbe_result* total);
 void gmbe_batch_destroy(gmbe_batch* b);
@@ -240,22 +261,30 @@ typedef struct {
   double t_submit_s, t_done_s; /* monotonic clock of submit / completion */
 } gmbe_serve_done;
 
-/* A server keeps `resident` (file orientation) on the device and runs
- * requests with continuous batching: requests are admitted while slots and
- * the device memory budget (bytes) allow, and their work overlaps on the GPU.
- * No reference counterpart. */
+/* A server keeps `resident` (file orientation; may be NULL) and runs requests
+ * with continuous batching on three lanes, each with its own stream and
+ * device buffers: whenever a lane is free, the waiting requests are admitted
+ * in arrival order while `slots` (requests per launch) and the lane's share
+ * of `mem_budget` (bytes of request data: CSR both ways, keys, table) allow,
+ * and launched as one multi-graph run that overlaps the runs of the other
+ * lanes.  A request that can never fit is rejected (GMBE_ERR_REJECTED) at
+ * submit.  No reference counterpart. */
 int gmbe_serve_create(const gmbe_graph* resident, int slots,
                       uint64_t mem_budget, int bound_height, int bound_size,
                       gmbe_server** out);
-/* Submit a stand-alone graph (copied; file orientation). */
+/* Submit a stand-alone graph (file orientation; copied and brought to the
+ * -t 1 -o 1 -f working form on the calling thread). */
 int gmbe_serve_submit_graph(gmbe_server* s, const gmbe_graph* g,
                             uint64_t* ticket);
-/* Submit the 2-hop ego subgraph of original-U vertex u of the resident graph. */
+/* Submit the 2-hop ego subgraph of original-U vertex u of the resident graph
+ * (gmbe_ego_subgraph with max_edges). */
 int gmbe_serve_submit_ego(gmbe_server* s, uint32_t u, uint64_t max_edges,
                           uint64_t* ticket);
-/* Collect finished requests (non-blocking unless wait != 0). */
+/* Collect up to max finished requests; with wait != 0, block until one is
+ * finished or none is outstanding. */
 int gmbe_serve_poll(gmbe_server* s, gmbe_serve_done* out, int max, int wait,
                     int* n_out);
+/* Finishes the requests on the GPU, drops those still waiting. */
 void gmbe_serve_destroy(gmbe_server* s);
 
 /* ---- misc ---------------------------------------------------------------- */

@@ -1,0 +1,108 @@
+"""Batched form: per-graph counts and hashes equal the plain finder's (which
+the finder parity tests tie to the oracle), across replays of the captured
+launch, the looped baseline, size-class thresholds and the autotuner."""
+import pytest
+
+import graphs
+import oracle
+from gmbe_amd import Batch, BiGraph, GmbeError, enumerate_graph, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def plain(g):
+    r = enumerate_graph(g)
+    assert r.overflow == 0
+    return r.digest()
+
+
+@pytest.fixture(scope="module")
+def small_set():
+    gs = workloads.batch_graphs(48)
+    # edge cases: closed-form graphs and a single edge
+    for name in ("crown_5", "K_3x4", "matching_7", "star_v_10"):
+        gs.append(graphs.kat_graph(name, uvtrans=1, order=1, fast_mode=1)[0])
+    gs.append(BiGraph.from_rows(1, [[0]], uvtrans=1, order=1, fast_mode=1))
+    return gs
+
+
+@pytest.fixture(scope="module")
+def expected(small_set):
+    return [plain(g) for g in small_set]
+
+
+def test_batch_matches_plain_finder(gpu, small_set, expected):
+    b = Batch(small_set)
+    per, tot = b.execute()
+    assert [r.digest() for r in per] == expected
+    assert tot.maximal == sum(e[0] for e in expected)
+    b.close()
+
+
+def test_oracle_spot_check(gpu, small_set):
+    b = Batch(small_set[:6])
+    per, _ = b.execute()
+    for g, r in zip(small_set[:6], per):
+        assert r.digest() == oracle.gmbe(g.oracle_graph(), threads=8).digest()
+    b.close()
+
+
+def test_replays_of_the_captured_launch_agree(gpu, small_set, expected):
+    b = Batch(small_set)
+    for i in range(4):
+        per, _ = b.execute()
+        assert [r.digest() for r in per] == expected, f"replay {i}"
+        # the program's own eager launches and host waits between replays
+        assert plain(small_set[i]) == expected[i]
+    b.close()
+
+
+def test_looped_baseline_matches(gpu, small_set, expected):
+    b = Batch(small_set, per_graph_launch=1)
+    per, _ = b.execute()
+    assert [r.digest() for r in per] == expected
+    b.close()
+
+
+@pytest.mark.parametrize("knobs", [
+    {"bound_height.0": 0, "bound_size.0": 0, "bound_height.1": 0, "bound_size.1": 0,
+     "bound_height.2": 0, "bound_size.2": 0, "bound_height.3": 0, "bound_size.3": 0},
+    {"share": 0}, {"lds_kb": 0}, {"waves_per_block": 8, "lds_kb": 4},
+    {"waves_per_block": 2, "lds_kb": 16}, {"group": 0},
+])
+def test_knobs_keep_per_graph_results(gpu, small_set, expected, knobs):
+    # (split=0 is left out: these graphs have hub roots whose unsplit subtrees
+    # run for minutes on one wave; the unsplit traversal is checked on config 1)
+    b = Batch(small_set)
+    for k, v in knobs.items():
+        b.set(k, v)
+    per, _ = b.execute()
+    assert [r.digest() for r in per] == expected
+    b.close()
+
+
+def test_autotuner_keeps_results(gpu, small_set, expected):
+    # graphs of every size class but the hub-heavy ones, so that the tuning
+    # trials stay short
+    pick = [i for i, g in enumerate(small_set) if g.stats()["edge_size"] < 16000]
+    gs = [small_set[i] for i in pick]
+    b = Batch(gs)
+    b.set("autotune", 1)
+    assert b.get("waves_per_block") in (2, 4, 8)
+    assert b.get("lds_kb") in (0, 4, 8, 16)
+    for c in range(b.get("n_classes")):
+        assert b.get(f"bound_height.{c}") >= 0
+    per, _ = b.execute()
+    assert [r.digest() for r in per] == [expected[i] for i in pick]
+    b.close()
+
+
+def test_bad_knobs_are_errors(gpu, small_set):
+    b = Batch(small_set[:2])
+    with pytest.raises(GmbeError):
+        b.set("waves_per_block", 3)
+    with pytest.raises(GmbeError):
+        b.set("bound_height.9", 1)
+    with pytest.raises(GmbeError):
+        b.set("nonsense", 1)
+    b.close()

@@ -1,0 +1,83 @@
+"""Serving loop: every request's count and hash equal the plain finder's on
+the same request graph, whether it ran alone or in a mix of overlapping
+launches; admission and error paths."""
+import numpy as np
+import pytest
+
+from gmbe_amd import GmbeError, Server, enumerate_graph, workloads
+
+pytestmark = pytest.mark.gpu
+RUN = workloads.RUN_OPTION
+
+
+@pytest.fixture(scope="module")
+def resident():
+    return workloads.raw_graph(2, 0.1)
+
+
+def alone(g_raw):
+    r = enumerate_graph(g_raw.derive(**RUN))
+    assert r.overflow == 0
+    return r.digest()
+
+
+def drain(s, n):
+    got = {}
+    while len(got) < n:
+        for d in s.poll(wait=True):
+            got[d.ticket] = d
+    return got
+
+
+def test_mixed_requests_equal_alone(gpu, resident):
+    rng = np.random.default_rng(5)
+    nU = resident.stats()["U_size"]
+    s = Server(resident, slots=64, mem_budget=1 << 28)
+    want = {}
+    for j in range(40):
+        if j % 2 == 0:
+            u = int(rng.integers(0, nU))
+            t = s.submit_ego(u, 20000)
+            want[t] = alone(resident.ego(u, 20000))
+        else:
+            g = workloads.small_graph(workloads.serve_small_seed(j))
+            t = s.submit_graph(g)
+            want[t] = alone(g)
+    got = drain(s, len(want))
+    s.close()
+    for t, d in got.items():
+        assert d.status == 0
+        assert (d.maximal, d.hash_sum, d.hash_xor) == want[t]
+        assert d.t_done_s >= d.t_submit_s
+
+
+def test_tight_budget_admits_one_at_a_time(gpu, resident):
+    gs = [workloads.small_graph(workloads.serve_small_seed(100 + j)) for j in range(6)]
+    biggest = max(g.stats()["edge_size"] for g in gs)
+    # room for about one request per lane
+    s = Server(None, slots=2, mem_budget=3 * (24 * biggest + 400_000))
+    want = {}
+    for g in gs:
+        want[s.submit_graph(g)] = alone(g)
+    got = drain(s, len(want))
+    s.close()
+    for t, d in got.items():
+        assert d.status == 0
+        assert (d.maximal, d.hash_sum, d.hash_xor) == want[t]
+
+
+def test_zero_budget_rejects(gpu):
+    s = Server(None, slots=4, mem_budget=0)
+    with pytest.raises(GmbeError) as e:
+        s.submit_graph(workloads.small_graph(workloads.serve_small_seed(1)))
+    assert e.value.code == -7
+    s.close()
+
+
+def test_ego_needs_a_resident_graph(gpu):
+    s = Server(None, slots=4, mem_budget=1 << 24)
+    with pytest.raises(GmbeError) as e:
+        s.submit_ego(0, 100)
+    assert e.value.code == -9
+    assert s.poll() == []
+    s.close()
