@@ -89,6 +89,8 @@ SIGNATURES = {
     "gmbe_batch_execute": (C.c_int, [_P, C.POINTER(Result), C.POINTER(Result)]),
     "gmbe_batch_destroy": (None, [_P]),
     "gmbe_serve_create": (C.c_int, [_P, C.c_int, C.c_uint64, C.c_int, C.c_int, _PP]),
+    "gmbe_serve_set": (C.c_int, [_P, C.c_char_p, C.c_int64]),
+    "gmbe_serve_stats": (C.c_int, [_P, C.POINTER(C.c_uint64), C.c_int]),
     "gmbe_serve_submit_graph": (C.c_int, [_P, _P, C.POINTER(C.c_uint64)]),
     "gmbe_serve_submit_ego": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64)]),
     "gmbe_serve_poll": (C.c_int, [_P, C.POINTER(ServeDone), C.c_int, C.c_int,

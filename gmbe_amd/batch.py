@@ -64,6 +64,15 @@ class Server:
                                       mem_budget, bound_height, bound_size, C.byref(out)))
         self._h = out
 
+    def set(self, key: str, value: int):
+        check(lib().gmbe_serve_set(self._h, key.encode(), int(value)))
+
+    def stats(self) -> dict:
+        out = (C.c_uint64 * 8)()
+        n = lib().gmbe_serve_stats(self._h, out, 8)
+        names = ["launches", "launched", "host_us", "gpu_us", "waiting"]
+        return {k: int(out[i]) for i, k in enumerate(names[:n])}
+
     def submit_graph(self, g) -> int:
         t = C.c_uint64()
         check(lib().gmbe_serve_submit_graph(self._h, g.handle, C.byref(t)))

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -39,6 +40,7 @@ struct gmbe_batch {
   LanePack pack;
   bool packed = false;       // the lane holds the current pack and thresholds
   int64_t per_graph_launch = 0;  // 1: one launch per graph (the plain-kernel loop)
+  int64_t verbose = 0;           // 1: the autotuner reports each trial on stderr
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool captured = false;
@@ -142,6 +144,8 @@ int run_looped(gmbe_batch* b, std::vector<gmbe_result>& out) {
     (void)hipEventElapsedTime(&ms, b->lane.ev0, b->lane.ev1);
     kern += ms * 1e-3;
     b->lane.result(0, &out[i]);
+    if (!b->lane.complete(0))
+      return set_error(GMBE_ERR_STATE, "graph %zu has unfinished work after the run", i);
   }
   b->last_wall_s = now_s() - t0;
   b->last_kernel_s = kern;
@@ -191,7 +195,7 @@ int timed(gmbe_batch* b, const std::vector<size_t>& sample, int reps, double bud
 int autotune(gmbe_batch* b) {
   // a stratified sample: up to kSample graphs of each size class, spread
   // over the class's members
-  constexpr size_t kSample = 24;
+  constexpr size_t kSample = 8;
   std::vector<size_t> sample;
   for (int c = 0; c < kClasses; ++c) {
     std::vector<size_t> mem;
@@ -210,6 +214,11 @@ int autotune(gmbe_batch* b) {
     double s = 0;
     rc = timed(b, sample, 2, budget(best), &s);
     if (rc) return rc;
+    if (b->verbose) {
+      fprintf(stderr, "autotune: %zu graphs, waves_per_block=%d lds_kb=%d: %.4f s\n",
+              sample.size(), v[0], v[1], s);
+      fflush(stderr);
+    }
     if (s < best) {
       best = s;
       bw = v[0];
@@ -230,6 +239,11 @@ int autotune(gmbe_batch* b) {
       double s = 0;
       rc = timed(b, sample, 2, budget(std::min(cbest, best)), &s);
       if (rc) return rc;
+      if (b->verbose) {
+        fprintf(stderr, "autotune: class %d bound_height=%d bound_size=%d: %.4f s\n", c, t[0],
+                t[1], s);
+        fflush(stderr);
+      }
       if (s < cbest) {
         cbest = s;
         bh = t[0];
@@ -310,6 +324,11 @@ int gmbe_batch_set(gmbe_batch* b, const char* key, int64_t value) {
     if ((rc = range(0, 1))) return rc;
     return value ? autotune(b) : GMBE_OK;
   }
+  if (k == "verbose") {
+    if ((rc = range(0, 1))) return rc;
+    b->verbose = value;
+    return GMBE_OK;
+  }
   if (k == "per_graph_launch") {
     if ((rc = range(0, 1))) return rc;
     b->per_graph_launch = value;
@@ -384,7 +403,12 @@ int gmbe_batch_execute(gmbe_batch* b, gmbe_result* per_graph, gmbe_result* total
     rc = run_captured(b);
     if (rc) return rc;
     res.resize(b->graphs.size());
-    for (size_t i = 0; i < b->graphs.size(); ++i) b->lane.result((uint32_t)i, &res[i]);
+    for (size_t i = 0; i < b->graphs.size(); ++i) {
+      b->lane.result((uint32_t)i, &res[i]);
+      // every unit of every graph accounted for: a run without deadline ends complete
+      if (!b->lane.complete((uint32_t)i))
+        return set_error(GMBE_ERR_STATE, "graph %zu has unfinished work after the run", i);
+    }
   }
   gmbe_result t{};
   for (const gmbe_result& r : res) {

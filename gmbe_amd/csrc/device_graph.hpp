@@ -56,6 +56,8 @@ hipError_t launch_persistent(const KernelChoice& kc, int blocks, hipStream_t st,
                              const mbe::Params& P);
 hipError_t launch_init_queue(hipStream_t st, unsigned long long* seq, unsigned long long cap);
 hipError_t launch_reset_ctx(hipStream_t st, const mbe::Params& P, unsigned long long n);
+// Per-graph outstanding units of a batch = each graph's top-level vertices.
+hipError_t launch_init_left(hipStream_t st, const mbe::Params& P, unsigned long long n);
 hipError_t launch_fill(hipStream_t st, unsigned long long* p, unsigned long long n,
                        unsigned long long v);
 // Control block for a new launch: zero, then work_left = n_roots.

@@ -76,6 +76,7 @@ void mbe_persistent(Params P) {
     }
     if (P.per_graph) flush_graph(P, c, gid);
     work_done(P);
+    graph_work(P, gid, -1);
   }
   if (lane_id() == 0) {
     unsigned long long* a = P.acc;
@@ -138,6 +139,13 @@ __global__ void reset_ctx(Params P, unsigned long long n) {
     ctx_hdr(P, i)->refs = 0ull;
 }
 
+// Outstanding units of every graph of a batch: its top-level vertices.
+__global__ void init_left(Params P, unsigned long long n) {
+  for (unsigned long long g = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; g < n;
+       g += (unsigned long long)gridDim.x * blockDim.x)
+    P.per_graph[g * kPerGraph + G_LEFT] = P.gt[g].nV;
+}
+
 // Reset the queue's slot sequence numbers (slot i of every shard starts free
 // for position i).
 __global__ void init_queue(unsigned long long* seq, unsigned long long cap) {
@@ -162,10 +170,12 @@ __global__ void ctl_start(unsigned long long* ctl, unsigned long long words, uns
 }
 
 // Control block of the wide pass, from the deferred count the main pass left.
+// A main pass cut short by its deadline (work left) gets no wide pass: its
+// deferred tasks stay unfinished and their graphs incomplete.
 __global__ void ctl_wide(unsigned long long* ctl, unsigned long long words, unsigned long long wl,
                          unsigned long long nd, unsigned long long wn, unsigned long long cur,
                          unsigned long long cap) {
-  const unsigned long long n = ctl[nd] < cap ? ctl[nd] : cap;
+  const unsigned long long n = ctl[wl] != 0 ? 0ull : ctl[nd] < cap ? ctl[nd] : cap;
   const unsigned long long cursor = ctl[cur];
   __syncthreads();
   for (unsigned long long i = threadIdx.x; i < words; i += blockDim.x) {
@@ -226,6 +236,10 @@ hipError_t launch_persistent(const KernelChoice& kc, int blocks, hipStream_t st,
 }
 hipError_t launch_init_queue(hipStream_t st, unsigned long long* seq, unsigned long long cap) {
   hipLaunchKernelGGL(mbe::init_queue, dim3(256), dim3(256), 0, st, seq, cap);
+  return hipGetLastError();
+}
+hipError_t launch_init_left(hipStream_t st, const mbe::Params& P, unsigned long long n) {
+  hipLaunchKernelGGL(mbe::init_left, dim3(64), dim3(256), 0, st, P, n);
   return hipGetLastError();
 }
 hipError_t launch_reset_ctx(hipStream_t st, const mbe::Params& P, unsigned long long n) {

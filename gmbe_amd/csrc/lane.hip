@@ -48,28 +48,47 @@ uint32_t LanePack::add(const gmbe_graph& g, int bh, int bs) {
   kU.insert(kU.end(), ku.begin(), ku.end());
   kV.insert(kV.end(), kv.begin(), kv.end());
   table.push_back(d);
+  double c = 0;
+  for (uint32_t u = 0; u < g.nU; ++u) {
+    const double du = (double)(g.u_off[u + 1] - g.u_off[u]);
+    c += du * du;
+  }
+  cost.push_back(c);
   max_nV = std::max(max_nV, g.nV);
   max_dmaxV = std::max(max_dmaxV, dmax);
   return id;
 }
 
-void LanePack::finish() {
-  const uint32_t n = (uint32_t)table.size();
-  std::vector<double> cost(n, 0.0);
-  for (uint32_t i = 0; i < n; ++i) {
-    const mbe::DevGraph& d = table[i];
-    const int64_t* uo = uoff.data() + (uintptr_t)d.u_off;
-    double c = 0;
-    for (uint32_t u = 0; u < d.nU; ++u) {
-      const double du = (double)(uo[u + 1] - uo[u]);
-      c += du * du;
-    }
-    cost[i] = c;
+void LanePack::append(const LanePack& o) {
+  const uintptr_t bvo = voff.size(), bva = vadj.size(), buo = uoff.size(), bua = uadj.size(),
+                  bku = kU.size(), bkv = kV.size();
+  for (mbe::DevGraph d : o.table) {
+    d.v_off = (const int64_t*)((uintptr_t)d.v_off + bvo);
+    d.v_adj = (const uint32_t*)((uintptr_t)d.v_adj + bva);
+    d.u_off = (const int64_t*)((uintptr_t)d.u_off + buo);
+    d.u_adj = (const uint32_t*)((uintptr_t)d.u_adj + bua);
+    d.kU = (const uint64_t*)((uintptr_t)d.kU + bku);
+    d.kV = (const uint64_t*)((uintptr_t)d.kV + bkv);
+    table.push_back(d);
   }
+  cost.insert(cost.end(), o.cost.begin(), o.cost.end());
+  voff.insert(voff.end(), o.voff.begin(), o.voff.end());
+  uoff.insert(uoff.end(), o.uoff.begin(), o.uoff.end());
+  vadj.insert(vadj.end(), o.vadj.begin(), o.vadj.end());
+  uadj.insert(uadj.end(), o.uadj.begin(), o.uadj.end());
+  kU.insert(kU.end(), o.kU.begin(), o.kU.end());
+  kV.insert(kV.end(), o.kV.begin(), o.kV.end());
+  max_nV = std::max(max_nV, o.max_nV);
+  max_dmaxV = std::max(max_dmaxV, o.max_dmaxV);
+}
+
+void LanePack::finish(bool light_first) {
+  const uint32_t n = (uint32_t)table.size();
   order.resize(n);
   for (uint32_t i = 0; i < n; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    return light_first ? cost[a] < cost[b] : cost[a] > cost[b];
+  });
   root_base.assign(n + 1, 0ull);
   for (uint32_t j = 0; j < n; ++j) root_base[j + 1] = root_base[j] + table[order[j]].nV;
 }
@@ -82,6 +101,7 @@ void LanePack::clear() {
   kU.clear();
   kV.clear();
   table.clear();
+  cost.clear();
   order.clear();
   root_base.clear();
   max_nV = max_dmaxV = 0;
@@ -219,6 +239,89 @@ int Lane::upload(const LanePack& p) {
   return GMBE_OK;
 }
 
+int Lane::upload_many(const std::vector<const LanePack*>& parts, bool light_first) {
+  uint64_t nvo = 0, nuo = 0, ne = 0;
+  uint32_t n = 0, mv = 0, md = 0;
+  for (const LanePack* p : parts) {
+    nvo += p->voff.size();
+    nuo += p->uoff.size();
+    ne += p->vadj.size();
+    n += (uint32_t)p->table.size();
+    mv = std::max(mv, p->max_nV);
+    md = std::max(md, p->max_dmaxV);
+  }
+  if (n > caps.max_graphs || nvo > caps.max_voff || nuo > caps.max_uoff || ne > caps.max_edges ||
+      mv > caps.max_nV || md > caps.max_dmaxV)
+    return set_error(GMBE_ERR_CAPACITY, "graph set exceeds the lane's capacity");
+  // stage layout: voff | uoff | kU | kV | vadj | uadj | table | root_base | order
+  auto al = [](uint64_t b) { return (b + 15) / 16 * 16; };
+  char* s_voff = stage;
+  char* s_uoff = s_voff + al(nvo * 8);
+  char* s_kU = s_uoff + al(nuo * 8);
+  char* s_kV = s_kU + al(nuo * 8);
+  char* s_vadj = s_kV + al(nvo * 8);
+  char* s_uadj = s_vadj + al(ne * 4);
+  char* s_tab = s_uadj + al(ne * 4);
+  char* s_rb = s_tab + al((uint64_t)n * sizeof(mbe::DevGraph));
+  char* s_ord = s_rb + al(((uint64_t)n + 1) * 8);
+  mbe::DevGraph* tab = (mbe::DevGraph*)s_tab;
+  std::vector<double> cost;
+  cost.reserve(n);
+  uint64_t bvo = 0, buo = 0, bva = 0, bua = 0, bku = 0, bkv = 0;
+  uint32_t gi = 0;
+  auto cp = [](char* dst, const void* src, uint64_t bytes) {
+    if (bytes) memcpy(dst, src, bytes);
+  };
+  for (const LanePack* p : parts) {
+    for (size_t j = 0; j < p->table.size(); ++j) {
+      mbe::DevGraph d = p->table[j];
+      d.v_off = voff + bvo + (uintptr_t)d.v_off;
+      d.v_adj = vadj + bva + (uintptr_t)d.v_adj;
+      d.u_off = uoff + buo + (uintptr_t)d.u_off;
+      d.u_adj = uadj + bua + (uintptr_t)d.u_adj;
+      d.kU = kU + bku + (uintptr_t)d.kU;
+      d.kV = kV + bkv + (uintptr_t)d.kV;
+      tab[gi++] = d;
+      cost.push_back(p->cost[j]);
+    }
+    cp(s_voff + bvo * 8, p->voff.data(), p->voff.size() * 8);
+    cp(s_uoff + buo * 8, p->uoff.data(), p->uoff.size() * 8);
+    cp(s_kU + bku * 8, p->kU.data(), p->kU.size() * 8);
+    cp(s_kV + bkv * 8, p->kV.data(), p->kV.size() * 8);
+    cp(s_vadj + bva * 4, p->vadj.data(), p->vadj.size() * 4);
+    cp(s_uadj + bua * 4, p->uadj.data(), p->uadj.size() * 4);
+    bvo += p->voff.size();
+    buo += p->uoff.size();
+    bku += p->kU.size();
+    bkv += p->kV.size();
+    bva += p->vadj.size();
+    bua += p->uadj.size();
+  }
+  uint32_t* ord = (uint32_t*)s_ord;
+  for (uint32_t i = 0; i < n; ++i) ord[i] = i;
+  std::stable_sort(ord, ord + n, [&](uint32_t a, uint32_t b) {
+    return light_first ? cost[a] < cost[b] : cost[a] > cost[b];
+  });
+  unsigned long long* rb = (unsigned long long*)s_rb;
+  rb[0] = 0;
+  for (uint32_t j = 0; j < n; ++j) rb[j + 1] = rb[j] + tab[ord[j]].nV;
+  auto h2d = [&](void* dst, const char* src, uint64_t bytes) -> hipError_t {
+    return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st) : hipSuccess;
+  };
+  LANE_TRY(h2d(voff, s_voff, nvo * 8));
+  LANE_TRY(h2d(uoff, s_uoff, nuo * 8));
+  LANE_TRY(h2d(kU, s_kU, nuo * 8));
+  LANE_TRY(h2d(kV, s_kV, nvo * 8));
+  LANE_TRY(h2d(vadj, s_vadj, ne * 4));
+  LANE_TRY(h2d(uadj, s_uadj, ne * 4));
+  LANE_TRY(h2d(gt, s_tab, (uint64_t)n * sizeof(mbe::DevGraph)));
+  LANE_TRY(h2d(root_base, s_rb, ((uint64_t)n + 1) * 8));
+  LANE_TRY(h2d(order, s_ord, (uint64_t)n * 4));
+  n_graphs = n;
+  n_roots = rb[n];
+  return GMBE_OK;
+}
+
 mbe::Params Lane::params(bool wide) const {
   mbe::Params P{};
   P.gt = gt;
@@ -270,6 +373,7 @@ int Lane::record_run() {
   LANE_TRY(launch_ctl_start(st, ctl, n_roots));
   LANE_TRY(launch_fill(st, acc, mbe::A_COUNT, 0ull));
   LANE_TRY(launch_fill(st, per_graph, ng * kPerGraph, 0ull));
+  if (n_graphs > 0) LANE_TRY(launch_init_left(st, P, n_graphs));
   LANE_TRY(launch_init_queue(st, qseq, qcap));
   LANE_TRY(launch_reset_ctx(st, P, (unsigned long long)nwaves * layout.ctx_slots));
   if (n_roots > 0) LANE_TRY(launch_persistent(kc, nblocks, st, P));
@@ -294,6 +398,10 @@ void Lane::result(uint32_t i, gmbe_result* r) const {
   r->overflow = a[mbe::G_OVERFLOW];
   r->alg_bytes = a[mbe::G_BALG];
   r->tasks = a[mbe::G_TASKS];
+}
+
+bool Lane::complete(uint32_t i) const {
+  return out_host[(uint64_t)i * kPerGraph + mbe::G_LEFT] == 0;
 }
 
 void Lane::totals(gmbe_result* r) const {

@@ -45,13 +45,17 @@ struct LanePack {
   std::vector<uint32_t> vadj, uadj;
   std::vector<uint64_t> kU, kV;
   std::vector<mbe::DevGraph> table;  // pointers are offsets until placed
+  std::vector<double> cost;          // claim-order cost proxy per graph
   std::vector<uint32_t> order;       // claim order of the graphs
   std::vector<unsigned long long> root_base;
   uint32_t max_nV = 0, max_dmaxV = 0;
   // add graph g with split thresholds (bh, bs); returns its index
   uint32_t add(const gmbe_graph& g, int bh, int bs);
-  // claim order: heaviest first (by sum of squared U degrees, a cost proxy)
-  void finish();
+  // append every graph of `o` (prepared elsewhere, e.g. on another thread)
+  void append(const LanePack& o);
+  // claim order by a cost proxy (sum of squared U degrees): heaviest first
+  // for throughput, lightest first when requests race a deadline
+  void finish(bool light_first = false);
   void clear();
   uint64_t total_roots() const { return root_base.empty() ? 0 : root_base.back(); }
 };
@@ -97,6 +101,9 @@ struct Lane {
   // Copy a pack into the lane (async on st; the pack must stay alive until
   // the stream has passed the copy).
   int upload(const LanePack& p);
+  // The same for the union of several packs (one graph each or more), copied
+  // straight into the staging buffer; claim order by the packs' cost proxy.
+  int upload_many(const std::vector<const LanePack*>& parts, bool light_first);
   // Record one full run on st: control reset, main pass, wide pass, results
   // to out_host.  No memset nodes, no host synchronisation: capturable.
   int record_run();
@@ -104,6 +111,8 @@ struct Lane {
   mbe::Params params(bool wide) const;
   // Per-graph result i (valid after the stream has passed record_run).
   void result(uint32_t i, gmbe_result* r) const;
+  // Whether graph i finished (false only for a run cut by its deadline).
+  bool complete(uint32_t i) const;
   // Totals of the last run.
   void totals(gmbe_result* r) const;
 };

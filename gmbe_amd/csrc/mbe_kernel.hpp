@@ -154,8 +154,17 @@ struct Counters {
 
 // Per-graph result counters of a batch (u64 each, kPerGraph per graph).
 enum PerGraph : int {
-  G_MAXIMAL = 0, G_HSUM, G_HXOR, G_NONMAX, G_NODES, G_OVERFLOW, G_BALG, G_TASKS, kPerGraph
+  G_MAXIMAL = 0, G_HSUM, G_HXOR, G_NONMAX, G_NODES, G_OVERFLOW, G_BALG, G_TASKS,
+  G_LEFT,  // units of the graph not yet finished (0 at the end: the graph is complete)
+  kPerGraph
 };
+
+// Count `delta` units of work of graph gid (batch only; lane 0).
+__device__ __forceinline__ void graph_work(const Params& P, uint32_t gid, long long delta) {
+  if (P.per_graph && lane_id() == 0)
+    __hip_atomic_fetch_add(P.per_graph + (uint64_t)gid * kPerGraph + G_LEFT,
+                           (unsigned long long)delta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Move this wave's result counters into graph gid's slots (batch only).
 __device__ __forceinline__ void flush_graph(const Params& P, Counters& c, uint32_t gid) {
@@ -889,6 +898,7 @@ __device__ bool publish_list(const Params& P, uint64_t gw, uint32_t ctx, uint32_
   if (lane == 0)
     __hip_atomic_fetch_add(s.work_left, (unsigned long long)ne, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
+  graph_work(P, gid, (long long)ne);
   // fill the entries: item j goes to entry j / 15, slot j % 15
   uint32_t j0 = 0;
   for (uint32_t base = 0; base < len; base += kWave) {
@@ -1231,8 +1241,10 @@ __device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC,
   uint32_t depth = 0, tick = 0;
   for (;;) {
     ++tick;
-    if (P.budget != 0 && (tick & 63u) == 0 && past_deadline(P)) {
+    // one wall-clock read per push: a push over a long list costs far more
+    if (P.budget != 0 && past_deadline(P)) {
       ctr.abandoned++;
+      graph_work(P, graph_id(P, m), 1);  // never finished: the graph stays incomplete
       return;
     }
     if (F.cur == kNone) {
@@ -1786,6 +1798,7 @@ __device__ void run_task(const Params& P, const WaveMem& m, const Task& t, int k
     const unsigned long long ne = (rest + kChunk - 1) / kChunk;
     if (lane == 0)
       __hip_atomic_fetch_add(P.s.work_left, ne, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    graph_work(P, t.gid, (long long)ne);
     uint32_t pushed = 0;
     if (queue_push_children(P.s, t, k, m.C + next_id, rest, pushed,
                             (uint32_t)(gw & (kShards - 1)))) {
@@ -1798,6 +1811,7 @@ __device__ void run_task(const Params& P, const WaveMem& m, const Task& t, int k
     if (lane == 0)
       __hip_atomic_fetch_add(P.s.work_left, (unsigned long long)(-(long long)ne),
                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    graph_work(P, t.gid, -(long long)ne);
   }
   if (must) {
     ctr.overflow++;
@@ -1825,6 +1839,7 @@ __device__ void run_task(const Params& P, const WaveMem& m, const Task& t, int k
     }
     if (lane < (uint32_t)kMaxPrefix) st_dev(&P.s.deferred[at].v[lane], t.v[lane]);
     if (lane == (uint32_t)kMaxPrefix) st_dev(&P.s.deferred[at].gid, t.gid);
+    graph_work(P, t.gid, 1);  // the wide pass finishes it
     return;
   }
   count_root(sumL, sumR);

@@ -19,7 +19,9 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -36,7 +38,9 @@ constexpr uint32_t kMaxSide = 1u << 16;  // per-request side size the slabs are 
 
 struct Request {
   uint64_t ticket = 0;
-  gmbe_graph* g = nullptr;  // working form, owned
+  std::shared_ptr<LanePack> one;  // the request in lane layout, built at submit
+  uint32_t nU = 0, nV = 0;
+  uint64_t E = 0;
   uint64_t bytes = 0;       // predicted device bytes
   double t_submit = 0;
 };
@@ -65,6 +69,10 @@ struct gmbe_server {
   std::deque<Request> pending;
   std::deque<gmbe_serve_done> done;
   uint64_t next_ticket = 1;
+  int64_t request_ms = 0;  // deadline of each launch (0 = none)
+  // statistics (server thread)
+  uint64_t launches = 0, launched = 0;
+  double host_s = 0, gpu_s = 0;
   uint64_t outstanding = 0;
   std::atomic<bool> stop{false};
   std::thread th;
@@ -76,6 +84,8 @@ namespace {
 void finish_lane(gmbe_server* s, int l) {
   Lane& L = s->lanes[l];
   const double t = now_s();
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, L.ev0, L.ev1) == hipSuccess) s->gpu_s += ms * 1e-3;
   std::vector<gmbe_serve_done> out;
   for (size_t i = 0; i < s->in_lane[l].size(); ++i) {
     Request& r = s->in_lane[l][i];
@@ -83,15 +93,14 @@ void finish_lane(gmbe_server* s, int l) {
     L.result((uint32_t)i, &res);
     gmbe_serve_done d{};
     d.ticket = r.ticket;
-    d.status = res.overflow ? GMBE_ERR_CAPACITY : GMBE_OK;
+    d.status = res.overflow ? GMBE_ERR_CAPACITY : L.complete((uint32_t)i) ? GMBE_OK : GMBE_ERR_DEADLINE;
     d.maximal = res.maximal;
     d.hash_sum = res.hash_sum;
     d.hash_xor = res.hash_xor;
     d.t_submit_s = r.t_submit;
     d.t_done_s = t;
     out.push_back(d);
-    gmbe_graph_destroy(r.g);
-    r.g = nullptr;
+    r.one.reset();
   }
   s->in_lane[l].clear();
   s->busy[l] = false;
@@ -103,11 +112,20 @@ void finish_lane(gmbe_server* s, int l) {
 
 int launch_lane(gmbe_server* s, int l, std::vector<Request>& batch) {
   Lane& L = s->lanes[l];
-  s->pack.clear();
-  for (Request& r : batch) s->pack.add(*r.g, s->bh, s->bs);
-  s->pack.finish();
-  int rc = L.upload(s->pack);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    L.knobs.budget_ms = s->request_ms;
+  }
+  const double t0 = now_s();
+  std::vector<const LanePack*> parts;
+  for (Request& r : batch) parts.push_back(r.one.get());
+  // with a deadline, the cheap requests go first so that most of them finish
+  int rc = L.upload_many(parts, L.knobs.budget_ms > 0);
   if (rc) return rc;
+  if (hipEventRecord(L.ev0, L.st) != hipSuccess) return set_error(GMBE_ERR_HIP, "event record");
+  s->host_s += now_s() - t0;
+  s->launches++;
+  s->launched += batch.size();
   rc = L.record_run();
   if (rc) return rc;
   if (hipEventRecord(L.ev1, L.st) != hipSuccess) return set_error(GMBE_ERR_HIP, "event record");
@@ -136,12 +154,12 @@ void server_loop(gmbe_server* s) {
         uint64_t voff = 0, uoff = 0, edges = 0, bytes = 0;
         while (!s->pending.empty() && (int)batch.size() < s->slots) {
           const Request& r = s->pending.front();
-          if (bytes + r.bytes > s->lane_bytes || voff + r.g->nV + 1 > c.max_voff ||
-              uoff + r.g->nU + 1 > c.max_uoff || edges + r.g->E > c.max_edges)
+          if (bytes + r.bytes > s->lane_bytes || voff + r.nV + 1 > c.max_voff ||
+              uoff + r.nU + 1 > c.max_uoff || edges + r.E > c.max_edges)
             break;  // admission: wait for memory to free up
-          voff += r.g->nV + 1;
-          uoff += r.g->nU + 1;
-          edges += r.g->E;
+          voff += r.nV + 1;
+          uoff += r.nU + 1;
+          edges += r.E;
           bytes += r.bytes;
           batch.push_back(s->pending.front());
           s->pending.pop_front();
@@ -158,7 +176,6 @@ void server_loop(gmbe_server* s) {
           d.t_submit_s = r.t_submit;
           d.t_done_s = now_s();
           s->done.push_back(d);
-          gmbe_graph_destroy(r.g);
           --s->outstanding;
         }
         s->cv_done.notify_all();
@@ -187,9 +204,11 @@ void server_loop(gmbe_server* s) {
 
 int submit(gmbe_server* s, gmbe_graph* w, uint64_t* ticket) {
   Request r;
-  r.g = w;
   r.bytes = predict_bytes(*w);
   r.t_submit = now_s();
+  r.nU = w->nU;
+  r.nV = w->nV;
+  r.E = w->E;
   const LaneCaps& c = s->lanes[0].caps;
   if (r.bytes > s->lane_bytes || w->nV + 1 > c.max_voff || w->nU + 1 > c.max_uoff ||
       w->E > c.max_edges || w->nV > c.max_nV || w->nU >= kMaxSide) {
@@ -205,6 +224,10 @@ int submit(gmbe_server* s, gmbe_graph* w, uint64_t* ticket) {
     return set_error(GMBE_ERR_REJECTED, "request degree %u exceeds the server's %u", dmax,
                      c.max_dmaxV);
   }
+  // the lane layout (CSR copies, hash keys) is built here, on the caller's thread
+  r.one = std::make_shared<LanePack>();
+  r.one->add(*w, s->bh, s->bs);
+  gmbe_graph_destroy(w);
   {
     std::lock_guard<std::mutex> lk(s->mu);
     r.ticket = s->next_ticket++;
@@ -248,7 +271,7 @@ int gmbe_serve_create(const gmbe_graph* resident, int slots, uint64_t mem_budget
   caps.max_nV = kMaxSide;
   caps.max_dmaxV = kMaxSide;
   LaneKnobs kn;
-  kn.arena_kb = 1024;
+  kn.arena_kb = 4096;
   kn.ctx_kb = 512;
   kn.queue_log2 = 18;
   kn.deferred_cap = 1ull << 16;
@@ -267,6 +290,30 @@ int gmbe_serve_create(const gmbe_graph* resident, int slots, uint64_t mem_budget
   s->th = std::thread(server_loop, s);
   *out = s;
   return GMBE_OK;
+}
+
+int gmbe_serve_set(gmbe_server* s, const char* key, int64_t value) {
+  if (!s || !key) return set_error(GMBE_ERR_ARG, "null argument");
+  const std::string k(key);
+  if (k == "request_ms") {
+    if (value < 0 || value > (1ll << 32))
+      return set_error(GMBE_ERR_ARG, "request_ms=%lld out of range", (long long)value);
+    // read by the server thread when it records the next launch of each lane
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->request_ms = value;
+    return GMBE_OK;
+  }
+  return set_error(GMBE_ERR_ARG, "unknown knob '%s'", key);
+}
+
+int gmbe_serve_stats(gmbe_server* s, uint64_t* out, int n) {
+  if (!s || (n > 0 && !out)) return set_error(GMBE_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  const uint64_t v[5] = {s->launches, s->launched, (uint64_t)(s->host_s * 1e6),
+                         (uint64_t)(s->gpu_s * 1e6), (uint64_t)s->pending.size()};
+  const int k = n < 5 ? n : 5;
+  for (int i = 0; i < k; ++i) out[i] = v[i];
+  return k;
 }
 
 int gmbe_serve_submit_graph(gmbe_server* s, const gmbe_graph* g, uint64_t* ticket) {
@@ -316,7 +363,6 @@ void gmbe_serve_destroy(gmbe_server* s) {
   for (int l = 0; l < kLanes; ++l) s->lanes[l].release();
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    for (Request& r : s->pending) gmbe_graph_destroy(r.g);
     s->pending.clear();
   }
   if (s->resident) gmbe_graph_destroy(s->resident);

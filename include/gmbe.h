@@ -239,7 +239,9 @@ int gmbe_batch_create(const gmbe_graph* const* graphs, int n, int bound_height,
  * "per_graph_launch" = 1 runs the graphs one launch (and one host wait) at a
  * time with the same kernel, the looped baseline;
  * "autotune" = 1 measures the kernel variants and then each class's
- * threshold candidates on this batch and keeps the fastest (synchronous). */
+ * threshold candidates on a stratified sample of this batch (8 graphs per
+ * class) and keeps the fastest (synchronous); "verbose" = 1 makes it print
+ * one line per trial on stderr. */
 int gmbe_batch_set(gmbe_batch* b, const char* key, int64_t value);
 /* Read back a knob or a choice of the autotuner: "waves_per_block",
  * "lds_kb", "n_classes", "bound_height.<c>", "bound_size.<c>", "class.<i>"
@@ -272,6 +274,10 @@ typedef struct {
 int gmbe_serve_create(const gmbe_graph* resident, int slots,
                       uint64_t mem_budget, int bound_height, int bound_size,
                       gmbe_server** out);
+/* Knob: "request_ms" (default 0 = none): deadline of each launch; requests
+ * of a launch that reaches it finish with status GMBE_ERR_DEADLINE and
+ * partial counts, the others with their exact result. */
+int gmbe_serve_set(gmbe_server* s, const char* key, int64_t value);
 /* Submit a stand-alone graph (file orientation; copied and brought to the
  * -t 1 -o 1 -f working form on the calling thread). */
 int gmbe_serve_submit_graph(gmbe_server* s, const gmbe_graph* g,
@@ -284,6 +290,11 @@ int gmbe_serve_submit_ego(gmbe_server* s, uint32_t u, uint64_t max_edges,
  * finished or none is outstanding. */
 int gmbe_serve_poll(gmbe_server* s, gmbe_serve_done* out, int max, int wait,
                     int* n_out);
+/* Server statistics, up to n values: [0] launches, [1] requests launched,
+ * [2] microseconds the server thread spent packing and uploading, [3]
+ * microseconds of launches on the GPU (events, summed over lanes), [4]
+ * requests waiting.  Returns the number of values written. */
+int gmbe_serve_stats(gmbe_server* s, uint64_t* out, int n);
 /* Finishes the requests on the GPU, drops those still waiting. */
 void gmbe_serve_destroy(gmbe_server* s);
 

@@ -81,3 +81,23 @@ def test_ego_needs_a_resident_graph(gpu):
     assert e.value.code == -9
     assert s.poll() == []
     s.close()
+
+
+def test_request_deadline_returns_exact_or_cut(gpu, resident):
+    """With a per-launch deadline every request either finishes exactly or is
+    reported as cut short (GMBE_ERR_DEADLINE); none is silently wrong."""
+    gs = [workloads.small_graph(workloads.batch_seed(i)) for i in range(24)]
+    s = Server(None, slots=8, mem_budget=1 << 28)
+    s.set("request_ms", 5)
+    want = {}
+    for g in gs:
+        want[s.submit_graph(g)] = alone(g)
+    got = drain(s, len(want))
+    s.close()
+    statuses = {d.status for d in got.values()}
+    assert statuses <= {0, -10}
+    for t, d in got.items():
+        if d.status == 0:
+            assert (d.maximal, d.hash_sum, d.hash_xor) == want[t]
+        else:
+            assert d.maximal <= want[t][0]
