@@ -38,6 +38,23 @@ def _hash_py(A, B):
     return _mix64(sa ^ rot)
 
 
+def test_mix64_is_splitmix64():
+    """mix64(x) is splitmix64's output for state x + golden gamma: the first
+    three outputs of splitmix64 seeded with 0 are the published
+    0xe220a8397b1dcdaf, 0x6e789e6aa1b965f4, 0x06c45d188009454f."""
+    g = 0x9E3779B97F4A7C15
+    assert _mix64(0) == 0xE220A8397B1DCDAF
+    assert _mix64(g) == 0x6E789E6AA1B965F4
+    assert _mix64((2 * g) & M64) == 0x06C45D188009454F
+    # the engine and the oracle use the same function: a one-element
+    # biclique's hash is mix64(mix64(a << 1) ^ rotl64(mix64(b << 1 | 1), 31))
+    for a, b in [(0, 0), (7, 3), (123456, 654321)]:
+        sb = _mix64((b << 1) | 1)
+        want = _mix64(_mix64(a << 1) ^ (((sb << 31) | (sb >> 33)) & M64))
+        assert oracle.hash_pair([a], [b]) == want
+        assert hash_biclique([a], [b]) == want
+
+
 def test_hash_known_answers():
     kat = json.load(open(os.path.join(HERE, "golden", "hash_kat.json")))
     for case in kat:
