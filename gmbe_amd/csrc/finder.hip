@@ -387,7 +387,7 @@ struct gmbe_finder {
   const gmbe_graph* g = nullptr;
   int sel = 2, bound_height = 20, bound_size = 1500;
   // knobs
-  int64_t waves_per_block = 4, blocks = 0, prune = 1, max_prefix = 4, lds_kb = 8;
+  int64_t waves_per_block = 4, blocks = 0, prune = 1, max_prefix = 4, lds_kb = 16;
   int64_t queue_log2 = 22, list_cap = 0, arena_kb = 16384, root_stride = 1, root_phase = 0;
   int64_t root_begin = 0, root_end = -1;  // processing-order rank range, -1 = all
   int64_t deadline_ms = 0;                 // 0 = run to completion

@@ -674,62 +674,77 @@ struct PassOut {
 template <int W, bool LIST_REMOTE>
 __device__ PassOut push_pass(Rec<W>* recs, uint32_t len, const uint64_t* ax, uint32_t xidx,
                              Rec<W>* crec, bool prune) {
+  // U chunks of 64 entries per iteration: all their loads are issued before
+  // the first is used, so that a wave keeps several list reads in flight
+  constexpr int U = W == 1 ? 4 : (W == 2 ? 2 : 1);
   const uint32_t lane = lane_id();
   bool nonmax = false;
   uint64_t absorbed = 0, best_next = ~0ull, best_child = ~0ull;
   uint32_t clen = 0, cbranch = 0;
-  for (uint32_t base = 0; base < len; base += kWave) {
-    const uint32_t k = base + lane;
-    const bool valid = k < len;
-    uint32_t mi = kNone, rep = kNone;
-    uint64_t me[W];
-    bool sub = true, zero = true, inside = true;
+  for (uint32_t base = 0; base < len; base += kWave * U) {
+    uint32_t mi[U], rp[U];
+    uint64_t ae[U][W], ks[U];
 #pragma unroll
-    for (int w = 0; w < W; ++w) me[w] = 0;
-    if (valid) {
-      mi = rd<LIST_REMOTE>(&recs[k].minidx);
-      rep = rd<LIST_REMOTE>(&recs[k].rep);
+    for (int u = 0; u < U; ++u) {
+      const uint32_t k = base + u * kWave + lane;
+      mi[u] = kNone;
+      rp[u] = kNone;
+      ks[u] = 0;
+#pragma unroll
+      for (int w = 0; w < W; ++w) ae[u][w] = 0;
+      if (k < len) {
+        mi[u] = rd<LIST_REMOTE>(&recs[k].minidx);
+        rp[u] = rd<LIST_REMOTE>(&recs[k].rep);
+        ks[u] = rd<LIST_REMOTE>(&recs[k].ksum);
+#pragma unroll
+        for (int w = 0; w < W; ++w) ae[u][w] = rd<LIST_REMOTE>(&recs[k].a[w]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t k = base + u * kWave + lane;
+      const bool valid = k < len;
+      const uint32_t rep = rp[u];
+      uint64_t me[W];
+      bool sub = valid, zero = true, inside = valid;
 #pragma unroll
       for (int w = 0; w < W; ++w) {
-        uint64_t ae = rd<LIST_REMOTE>(&recs[k].a[w]);
-        me[w] = ae & ax[w];
+        me[w] = ae[u][w] & ax[w];
         sub = sub && (me[w] == ax[w]);
         zero = zero && (me[w] == 0);
-        inside = inside && ((ae & ~ax[w]) == 0);
+        inside = inside && ((ae[u][w] & ~ax[w]) == 0);
       }
-    }
-    uint64_t ks = 0;
-    if (valid) ks = rd<LIST_REMOTE>(&recs[k].ksum);
-    if (valid && sub) {
-      if (mi < xidx) nonmax = true;  // a twin or superset before x
-      else absorbed += ks;           // joins R'
-    }
-    const bool br = rep != kNone && rep > xidx;
-    const bool pruned = prune && valid && br && inside;
-    if constexpr (!LIST_REMOTE) {
-      if (pruned) recs[k].rep = kNone;
-    }
-    const bool keep = valid && !sub && !zero;
-    const uint32_t crep = br ? rep : kNone;  // branchability before this push's prune
-    const uint64_t km = ballot(keep);
-    const uint32_t pos = clen + lanes_below(km);
-    if (keep) {
-      crec[pos].minidx = mi;
-      crec[pos].rep = crep;
-      crec[pos].ksum = ks;
+      if (sub) {
+        if (mi[u] < xidx) nonmax = true;  // a twin or superset before x
+        else absorbed += ks[u];           // joins R'
+      }
+      const bool br = rep != kNone && rep > xidx;
+      const bool pruned = prune && valid && br && inside;
+      if constexpr (!LIST_REMOTE) {
+        if (pruned) recs[k].rep = kNone;
+      }
+      const bool keep = valid && !sub && !zero;
+      const uint32_t crep = br ? rep : kNone;  // branchability before this push's prune
+      const uint64_t km = ballot(keep);
+      const uint32_t pos = clen + lanes_below(km);
+      if (keep) {
+        crec[pos].minidx = mi[u];
+        crec[pos].rep = crep;
+        crec[pos].ksum = ks[u];
 #pragma unroll
-      for (int w = 0; w < W; ++w) crec[pos].a[w] = me[w];
-      if (crep != kNone) {
-        uint64_t c = ((uint64_t)crep << 32) | pos;
-        best_child = c < best_child ? c : best_child;
+        for (int w = 0; w < W; ++w) crec[pos].a[w] = me[w];
+        if (crep != kNone) {
+          uint64_t c = ((uint64_t)crep << 32) | pos;
+          best_child = c < best_child ? c : best_child;
+        }
       }
+      if (valid && br && !pruned) {
+        uint64_t c = ((uint64_t)rep << 32) | k;
+        best_next = c < best_next ? c : best_next;
+      }
+      clen += __popcll(km);
+      cbranch += __popcll(ballot(keep && crep != kNone));
     }
-    if (valid && br && !pruned) {
-      uint64_t c = ((uint64_t)rep << 32) | k;
-      best_next = c < best_next ? c : best_next;
-    }
-    clen += __popcll(km);
-    cbranch += __popcll(ballot(keep && crep != kNone));
   }
   PassOut o;
   o.cbranch = cbranch;
