@@ -392,9 +392,9 @@ struct gmbe_finder {
   int64_t root_begin = 0, root_end = -1;  // processing-order rank range, -1 = all
   int64_t deadline_ms = 0;                 // 0 = run to completion
   int64_t late_share = 32;                 // 0 = off
-  int64_t ctx_slots = 4;                   // shared-context slots per wave
+  int64_t ctx_slots = 8;                   // shared-context slots per wave
   int64_t profile = 0, group = 1, share = 1, ctx_kb = 4096, share_work = 4096, share_queue = -1;
-  int64_t merge_min = 256, merge_branch = 16;
+  int64_t merge_min = 256, merge_branch = 64;
   unsigned long long prof[mbe::A_COUNT] = {};
   gmbe::Engine eng;
   gmbe::DeviceGraph dg;
