@@ -395,6 +395,7 @@ struct gmbe_finder {
   int64_t ctx_slots = 8;                   // shared-context slots per wave
   int64_t profile = 0, group = 1, share = 1, ctx_kb = 4096, share_work = 4096, share_queue = -1;
   int64_t merge_min = 256, merge_branch = 64;
+  bool arena_set = false, ctx_set = false;  // sizes given by the caller
   unsigned long long prof[mbe::A_COUNT] = {};
   gmbe::Engine eng;
   gmbe::DeviceGraph dg;
@@ -428,8 +429,17 @@ static int finder_alloc(gmbe_finder* f) {
   KernelChoice kc = pick_kernel((int)f->lds_kb, 4, false);
   int blocks = f->blocks > 0 ? (int)f->blocks : resident_blocks(kc, f->eng.cu_count);
   int nwaves = blocks * kc.waves;
-  mbe::Scratch lay = scratch_layout(f->dg.nV, f->dg.dmaxV, (uint64_t)f->arena_kb * 1024ull,
-                                    (uint64_t)f->ctx_kb * 1024ull, (uint32_t)f->ctx_slots);
+  // unless set, the list arena and the context slots are sized for the graph:
+  // a list holds at most |V| entries of <= 60 bytes
+  const uint64_t nv = f->dg.nV;
+  const int64_t arena_kb = f->arena_set ? f->arena_kb
+                                        : std::min<int64_t>(16384, std::max<int64_t>(4096, (int64_t)(nv * 24 * 64 / 1024)));
+  const int64_t ctx_kb = f->ctx_set ? f->ctx_kb
+                                    : std::min<int64_t>(4096, std::max<int64_t>(512, (int64_t)(nv * 128 / 1024)));
+  f->arena_kb = arena_kb;
+  f->ctx_kb = ctx_kb;
+  mbe::Scratch lay = scratch_layout(f->dg.nV, f->dg.dmaxV, (uint64_t)arena_kb * 1024ull,
+                                    (uint64_t)ctx_kb * 1024ull, (uint32_t)f->ctx_slots);
   uint64_t bytes = lay.slab * (uint64_t)nwaves;
   if (bytes != f->scratch_bytes || nwaves != f->nwaves) {
     if (f->scratch) (void)hipFree(f->scratch);
@@ -532,14 +542,14 @@ int gmbe_finder_set(gmbe_finder* f, const char* key, int64_t value) {
   }
   else if (k == "queue_log2") { if ((rc = range(4, 30))) return rc; f->queue_log2 = value; }
   else if (k == "list_cap") { if ((rc = range(0, (int64_t)1 << 34))) return rc; f->list_cap = value; }
-  else if (k == "arena_kb") { if ((rc = range(64, 1 << 20))) return rc; f->arena_kb = value; }
+  else if (k == "arena_kb") { if ((rc = range(64, 1 << 20))) return rc; f->arena_kb = value; f->arena_set = true; }
   else if (k == "root_stride") { if ((rc = range(1, 1 << 30))) return rc; f->root_stride = value; }
   else if (k == "share_work") { if ((rc = range(1, 1ll << 40))) return rc; f->share_work = value; }
   else if (k == "share_queue") { if ((rc = range(-1, 1ll << 40))) return rc; f->share_queue = value; }
   else if (k == "merge_branch") { if ((rc = range(0, 1ll << 30))) return rc; f->merge_branch = value; }
   else if (k == "merge_min") { if ((rc = range(2, 1ll << 30))) return rc; f->merge_min = value; }
   else if (k == "share") { if ((rc = range(0, 1))) return rc; f->share = value; }
-  else if (k == "ctx_kb") { if ((rc = range(16, 1 << 20))) return rc; f->ctx_kb = value; }
+  else if (k == "ctx_kb") { if ((rc = range(16, 1 << 20))) return rc; f->ctx_kb = value; f->ctx_set = true; }
   else if (k == "group") { if ((rc = range(0, 1))) return rc; f->group = value; }
   else if (k == "profile") { if ((rc = range(0, 1))) return rc; f->profile = value; }
   else if (k == "ctx_slots") { if ((rc = range(1, 32))) return rc; f->ctx_slots = value; }

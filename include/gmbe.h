@@ -177,8 +177,9 @@ int gmbe_finder_create(const gmbe_graph* g, int sel, int bound_height,
  *   "waves_per_block", "blocks", "queue_log2", "list_cap" (u32 words of list
  *   output; 0 = off), "prune" (0/1, PRUNE_EN), "max_prefix" (2..8),
  *   "bound_height", "bound_size".
- * Scheduler and memory knobs (no counterpart): "lds_kb", "arena_kb",
- *   "ctx_kb", "ctx_slots", "group", "share", "share_work", "share_queue", "merge_min",
+ * Scheduler and memory knobs (no counterpart): "lds_kb", "arena_kb" and
+ *   "ctx_kb" (per wave; unless set they scale with |V|: 4-16 MB and
+ *   0.5-4 MB), "ctx_slots", "group", "share", "share_work", "share_queue", "merge_min",
  *   "merge_branch", "late_share", "profile".
  * "deadline_ms" (0 = none): every wave stops taking work, and abandons the
  *   subtree it is in, once it has run that long; execute then returns
