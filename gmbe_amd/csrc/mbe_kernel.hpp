@@ -679,6 +679,9 @@ __device__ PassOut push_pass(Rec<W>* recs, uint32_t len, const uint64_t* ax, uin
   constexpr int U = W == 1 ? 4 : (W == 2 ? 2 : 1);
   const uint32_t lane = lane_id();
   bool nonmax = false;
+  // once the node is known to be non-maximal its child list is never used:
+  // stop writing it (the prune marks and the next push still need every entry)
+  bool dead = false;
   uint64_t absorbed = 0, best_next = ~0ull, best_child = ~0ull;
   uint32_t clen = 0, cbranch = 0;
   for (uint32_t base = 0; base < len; base += kWave * U) {
@@ -723,7 +726,7 @@ __device__ PassOut push_pass(Rec<W>* recs, uint32_t len, const uint64_t* ax, uin
       if constexpr (!LIST_REMOTE) {
         if (pruned) recs[k].rep = kNone;
       }
-      const bool keep = valid && !sub && !zero;
+      const bool keep = !dead && valid && !sub && !zero;
       const uint32_t crep = br ? rep : kNone;  // branchability before this push's prune
       const uint64_t km = ballot(keep);
       const uint32_t pos = clen + lanes_below(km);
@@ -744,6 +747,7 @@ __device__ PassOut push_pass(Rec<W>* recs, uint32_t len, const uint64_t* ax, uin
       }
       clen += __popcll(km);
       cbranch += __popcll(ballot(keep && crep != kNone));
+      dead = dead || ballot(nonmax) != 0;
     }
   }
   PassOut o;
