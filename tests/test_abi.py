@@ -58,3 +58,32 @@ def test_multi_gpu_request_is_rejected():
     out = C.c_void_p()
     rc = _lib.lib().gmbe_finder_create(g.handle, 3, 20, 1500, 8, C.byref(out))
     assert rc in (-8, -5)
+
+
+def test_batch_and_server_without_gpu_fail_loudly():
+    """The batched form and the serving loop refuse to run without a GPU."""
+    from gmbe_amd import Batch, BiGraph, Server
+    n = C.c_int()
+    if _lib.lib().gmbe_device_info(None, 0, C.byref(n)) == 0:
+        pytest.skip("a GPU is visible")
+    g = BiGraph.from_rows(2, [[0, 1], [1]], uvtrans=1, order=1, fast_mode=1)
+    with pytest.raises(GmbeError) as e:
+        Batch([g])
+    assert e.value.code == -5
+    with pytest.raises(GmbeError) as e:
+        Server(None, slots=4, mem_budget=1 << 20)
+    assert e.value.code == -5
+
+
+def test_batch_and_server_argument_errors():
+    """Argument checks come before any device work."""
+    lib = _lib.lib()
+    out = C.c_void_p()
+    assert lib.gmbe_batch_create(None, 0, 20, 1500, C.byref(out)) == -1
+    arr = (C.c_void_p * 1)(None)
+    assert lib.gmbe_batch_create(arr, 1, 20, 1500, C.byref(out)) == -1
+    assert lib.gmbe_serve_create(None, 0, 1 << 20, 20, 1500, C.byref(out)) == -1
+    assert lib.gmbe_serve_create(None, 4, 1 << 20, -1, 1500, C.byref(out)) == -1
+    assert lib.gmbe_serve_poll(None, None, 0, 0, None) == -1
+    assert lib.gmbe_serve_set(None, b"request_ms", 1) == -1
+    assert lib.gmbe_batch_get(None, b"lds_kb", None) == -1
