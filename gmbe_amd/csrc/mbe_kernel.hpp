@@ -703,30 +703,40 @@ __device__ PassOut push_pass(Rec<W>* recs, uint32_t len, const uint64_t* ax, uin
         for (int w = 0; w < W; ++w) ae[u][w] = rd<LIST_REMOTE>(&recs[k].a[w]);
       }
     }
+    // maximality first, for the whole group: a group that shows the node is
+    // non-maximal writes none of its child entries
+    bool sub[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t k = base + u * kWave + lane;
+      sub[u] = k < len;
+#pragma unroll
+      for (int w = 0; w < W; ++w) sub[u] = sub[u] && ((ae[u][w] & ax[w]) == ax[w]);
+      if (sub[u]) {
+        if (mi[u] < xidx) nonmax = true;  // a twin or superset before x
+        else absorbed += ks[u];           // joins R'
+      }
+    }
+    dead = dead || ballot(nonmax) != 0;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t k = base + u * kWave + lane;
       const bool valid = k < len;
       const uint32_t rep = rp[u];
       uint64_t me[W];
-      bool sub = valid, zero = true, inside = valid;
+      bool zero = true, inside = valid;
 #pragma unroll
       for (int w = 0; w < W; ++w) {
         me[w] = ae[u][w] & ax[w];
-        sub = sub && (me[w] == ax[w]);
         zero = zero && (me[w] == 0);
         inside = inside && ((ae[u][w] & ~ax[w]) == 0);
-      }
-      if (sub) {
-        if (mi[u] < xidx) nonmax = true;  // a twin or superset before x
-        else absorbed += ks[u];           // joins R'
       }
       const bool br = rep != kNone && rep > xidx;
       const bool pruned = prune && valid && br && inside;
       if constexpr (!LIST_REMOTE) {
         if (pruned) recs[k].rep = kNone;
       }
-      const bool keep = !dead && valid && !sub && !zero;
+      const bool keep = !dead && valid && !sub[u] && !zero;
       const uint32_t crep = br ? rep : kNone;  // branchability before this push's prune
       const uint64_t km = ballot(keep);
       const uint32_t pos = clen + lanes_below(km);
@@ -747,7 +757,6 @@ __device__ PassOut push_pass(Rec<W>* recs, uint32_t len, const uint64_t* ax, uin
       }
       clen += __popcll(km);
       cbranch += __popcll(ballot(keep && crep != kNone));
-      dead = dead || ballot(nonmax) != 0;
     }
   }
   PassOut o;
