@@ -132,8 +132,10 @@ int Lane::alloc(Engine* e, const LaneCaps& c, const LaneKnobs& k) {
   LANE_TRY(hipMalloc(&ctl, kCtlWords * 8));
   LANE_TRY(hipMalloc(&acc, mbe::A_COUNT * 8));
   LANE_TRY(hipMalloc(&deferred, knobs.deferred_cap * sizeof(mbe::Task)));
+  // the arrays of a full lane plus the 16-byte alignment of each of the nine
+  // pieces an upload stages
   stage_bytes = (c.max_voff + c.max_uoff) * 16 + c.max_edges * 8 +
-                ng * (sizeof(mbe::DevGraph) + 8 + 4) + 8;
+                ng * (sizeof(mbe::DevGraph) + 8 + 4) + 8 + 9 * 16;
   LANE_TRY(hipHostMalloc((void**)&stage, stage_bytes, hipHostMallocDefault));
   LANE_TRY(hipHostMalloc((void**)&out_host, (ng * kPerGraph + mbe::A_COUNT) * 8,
                          hipHostMallocDefault));

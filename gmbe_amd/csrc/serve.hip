@@ -76,7 +76,7 @@ struct gmbe_server {
   uint64_t outstanding = 0;
   std::atomic<bool> stop{false};
   std::thread th;
-  int fatal = 0;
+  std::atomic<int> fatal{0};  // set by the server thread, read by submitters
 };
 
 namespace {
@@ -167,7 +167,7 @@ void server_loop(gmbe_server* s) {
       }
       if (batch.empty()) break;
       if (launch_lane(s, l, batch) != GMBE_OK) {
-        s->fatal = GMBE_ERR_HIP;
+        s->fatal.store(GMBE_ERR_HIP);
         std::lock_guard<std::mutex> lk(s->mu);
         for (Request& r : batch) {
           gmbe_serve_done d{};
@@ -318,7 +318,7 @@ int gmbe_serve_stats(gmbe_server* s, uint64_t* out, int n) {
 
 int gmbe_serve_submit_graph(gmbe_server* s, const gmbe_graph* g, uint64_t* ticket) {
   if (!s || !g) return set_error(GMBE_ERR_ARG, "null argument");
-  if (s->fatal) return set_error(s->fatal, "the server has failed");
+  if (s->fatal.load()) return set_error(s->fatal.load(), "the server has failed");
   gmbe_option run{1, 1, 1};
   gmbe_graph* w = nullptr;
   int rc = gmbe_graph_derive(g, &run, &w);
@@ -329,7 +329,7 @@ int gmbe_serve_submit_graph(gmbe_server* s, const gmbe_graph* g, uint64_t* ticke
 int gmbe_serve_submit_ego(gmbe_server* s, uint32_t u, uint64_t max_edges, uint64_t* ticket) {
   if (!s) return set_error(GMBE_ERR_ARG, "null argument");
   if (!s->resident) return set_error(GMBE_ERR_STATE, "the server has no resident graph");
-  if (s->fatal) return set_error(s->fatal, "the server has failed");
+  if (s->fatal.load()) return set_error(s->fatal.load(), "the server has failed");
   gmbe_graph* ego = nullptr;
   int rc = gmbe_ego_subgraph(s->resident, u, max_edges, &ego);
   if (rc) return rc;
