@@ -71,6 +71,7 @@ SIGNATURES = {
     "gmbe_graph_stats": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "gmbe_graph_csr": (C.c_int, [_P, _PP, _PP, _PP, _PP, _PP, _PP]),
     "gmbe_graph_write_adj": (C.c_int, [_P, C.c_char_p]),
+    "gmbe_graph_write_edges": (C.c_int, [_P, C.c_char_p]),
     "gmbe_graph_destroy": (None, [_P]),
     "gmbe_generate": (C.c_int, [C.POINTER(GenParams), _PP]),
     "gmbe_generate_small": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_double,

@@ -411,6 +411,45 @@ int gmbe_graph_write_adj(const gmbe_graph* g, const char* path) {
   return GMBE_OK;
 }
 
+int gmbe_graph_write_edges(const gmbe_graph* g, const char* path) {
+  if (!g || !path) return set_error(GMBE_ERR_ARG, "null argument");
+  // rows are the original V side, columns the original U side, in original ids
+  const bool sw = g->swapped != 0;
+  const uint32_t nrows = sw ? g->nU : g->nV, ncols = sw ? g->nV : g->nU;
+  const int64_t* off = sw ? g->u_off.data() : g->v_off.data();
+  const uint32_t* adj = sw ? g->u_adj.data() : g->v_adj.data();
+  const std::vector<uint32_t>& row_orig = sw ? g->u_orig : g->v_orig;
+  const std::vector<uint32_t>& col_orig = sw ? g->v_orig : g->u_orig;
+  std::vector<uint32_t> rows(nrows);
+  for (uint32_t i = 0; i < nrows; ++i) rows[i] = i;
+  std::sort(rows.begin(), rows.end(),
+            [&](uint32_t a, uint32_t b) { return row_orig[a] < row_orig[b]; });
+  FILE* fp = fopen(path, "wb");
+  if (!fp) return set_error(GMBE_ERR_IO, "can't open %s for writing", path);
+  std::string buf = std::to_string(nrows) + " " + std::to_string(ncols) + "\n";
+  std::vector<uint32_t> cols;
+  bool ok = true;
+  for (uint32_t i : rows) {
+    cols.clear();
+    for (int64_t e = off[i]; e < off[i + 1]; ++e) cols.push_back(col_orig[adj[e]]);
+    std::sort(cols.begin(), cols.end());
+    const std::string r = std::to_string(row_orig[i]) + " ";
+    for (uint32_t c : cols) {
+      buf += r;
+      buf += std::to_string(c);
+      buf.push_back('\n');
+    }
+    if (buf.size() > (1u << 20)) {
+      ok = ok && fwrite(buf.data(), 1, buf.size(), fp) == buf.size();
+      buf.clear();
+    }
+  }
+  ok = ok && fwrite(buf.data(), 1, buf.size(), fp) == buf.size();
+  fclose(fp);
+  if (!ok) return set_error(GMBE_ERR_IO, "write failed: %s", path);
+  return GMBE_OK;
+}
+
 void gmbe_graph_destroy(gmbe_graph* g) { delete g; }
 
 uint64_t gmbe_hash_biclique(const uint32_t* A, size_t nA, const uint32_t* B, size_t nB) {

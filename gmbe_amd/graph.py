@@ -75,6 +75,11 @@ class BiGraph:
     def write_adj(self, path: str):
         check(lib().gmbe_graph_write_adj(self._h, path.encode()))
 
+    def write_edges(self, path: str):
+        """Edge-list file ("rows cols" header, one "v u" line per edge) in the
+        original orientation and ids (preprocess/adj_to_graph.cpp:31-39)."""
+        check(lib().gmbe_graph_write_edges(self._h, path.encode()))
+
     # -- access ---------------------------------------------------------------
     @property
     def handle(self):

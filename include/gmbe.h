@@ -111,6 +111,14 @@ int gmbe_graph_csr(const gmbe_graph* g, const int64_t** v_off,
 /* Write the graph in .adj format (one line per internal V vertex). */
 int gmbe_graph_write_adj(const gmbe_graph* g, const char* path);
 
+/* Write the graph as an edge list in its original orientation and ids: a
+ * first line "rows cols" (original V count, original U count), then one line
+ * "v u" per edge, rows ascending and each row's columns ascending.  Replaces
+ * the .adj -> .graph converter of the reference (preprocess/adj_to_graph.cpp:
+ * 31-39); unlike it, rows are written as loaded (sorted, without duplicates)
+ * and the column count is the U side's size. */
+int gmbe_graph_write_edges(const gmbe_graph* g, const char* path);
+
 /* Replaces CSRBiGraph::Reset() of a host graph (src/BiGraph.cu:433-454). */
 void gmbe_graph_destroy(gmbe_graph* g);
 

@@ -116,3 +116,25 @@ def test_write_adj_roundtrip(tmp_path):
     out = tmp_path / "out.adj"
     g.write_adj(str(out))
     assert out.read_text() == "0 2\n1\n0 1 2\n"
+
+
+def test_write_edges_matches_the_adj_rows(tmp_path):
+    """The edge-list writer (preprocess/adj_to_graph.cpp:31-39): header
+    "rows cols", then the edges row by row in original ids; a derived graph
+    (side swap, reordering) writes the same file as its source."""
+    rows = [[3, 1], [0], [], [2, 5, 1]]  # |U| = 6 > |V| = 4: uvtrans=2 swaps
+    p = tmp_path / "g.adj"
+    p.write_text("\n".join(" ".join(map(str, r)) for r in rows) + "\n")
+    g = BiGraph.read_adj(str(p))
+    out = tmp_path / "g.graph"
+    g.write_edges(str(out))
+    lines = out.read_text().splitlines()
+    assert lines[0] == "4 6"
+    want = [f"{v} {u}" for v, r in enumerate(rows) for u in sorted(r)]
+    assert lines[1:] == want
+    for opt in (dict(uvtrans=1, order=1, fast_mode=1), dict(uvtrans=2, order=2, fast_mode=0)):
+        d = g.derive(**opt)
+        assert d.stats()["swapped"] == (1 if opt["uvtrans"] == 2 else 0)
+        out2 = tmp_path / "d.graph"
+        d.write_edges(str(out2))
+        assert out2.read_text() == out.read_text()
