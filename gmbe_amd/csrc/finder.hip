@@ -24,6 +24,7 @@ namespace mbe {
 template <int WAVES, int LDS_PER_WAVE, int WMAX, int OCC>
 __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
 void mbe_persistent(Params P) {
+  const uint64_t t_start = P.exit_stamps ? wall_clock64() : 0;
   if (P.budget != 0) P.deadline = wall_clock64() + P.budget;
   __shared__ __attribute__((aligned(16))) char lds[WAVES * (LDS_PER_WAVE > 0 ? LDS_PER_WAVE : 16)];
   __shared__ uint32_t s_off[WAVES][kWave + 1];
@@ -77,6 +78,15 @@ void mbe_persistent(Params P) {
     if (P.per_graph) flush_graph(P, c, gid);
     work_done(P);
     graph_work(P, gid, -1);
+  }
+  if (P.exit_stamps) {
+    // the reference's per-SM exit times (src/IterFinderGpu.cu:1038-1041): once
+    // every wave of the block is done, the block's start and exit stamps
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      P.exit_stamps[2 * blockIdx.x] = t_start;
+      P.exit_stamps[2 * blockIdx.x + 1] = wall_clock64();
+    }
   }
   if (lane_id() == 0) {
     unsigned long long* a = P.acc;
@@ -415,6 +425,9 @@ struct gmbe_finder {
   uint64_t list_words = 0;
   mbe::Task* deferred = nullptr;
   mbe::DevGraph* gt = nullptr;  // device graph table (one entry)
+  int64_t exit_stamps = 0;              // knob: record per-block exit times
+  unsigned long long* stamps = nullptr;  // 1 + nblocks: launch stamp, then per block
+  std::vector<double> exit_s;            // per-block exit, seconds after launch
   uint64_t deferred_cap = 1ull << 20;
   uint64_t last_deferred = 0;
   hipStream_t stream = nullptr;
@@ -553,6 +566,7 @@ int gmbe_finder_set(gmbe_finder* f, const char* key, int64_t value) {
   else if (k == "group") { if ((rc = range(0, 1))) return rc; f->group = value; }
   else if (k == "profile") { if ((rc = range(0, 1))) return rc; f->profile = value; }
   else if (k == "ctx_slots") { if ((rc = range(1, 32))) return rc; f->ctx_slots = value; }
+  else if (k == "exit_stamps") { if ((rc = range(0, 1))) return rc; f->exit_stamps = value; }
   else if (k == "late_share") { if ((rc = range(0, 1ll << 30))) return rc; f->late_share = value; }
   else if (k == "deadline_ms") { if ((rc = range(0, 1ll << 32))) return rc; f->deadline_ms = value; }
   else if (k == "root_begin") { if ((rc = range(0, (1ll << 32)))) return rc; f->root_begin = value; }
@@ -626,6 +640,14 @@ int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res) {
   P.merge_min = (int)f->merge_min;
   P.merge_branch = (int)f->merge_branch;
   P.budget = (uint64_t)f->deadline_ms * (uint64_t)f->eng.wall_khz;
+  P.exit_stamps = nullptr;
+  if (f->exit_stamps) {
+    if (f->stamps) (void)hipFree(f->stamps);
+    f->stamps = nullptr;
+    HIP_TRY(hipMalloc(&f->stamps, 2ull * f->nblocks * sizeof(unsigned long long)));
+    HIP_TRY(hipMemsetAsync(f->stamps, 0, 2ull * f->nblocks * sizeof(unsigned long long), f->stream));
+    P.exit_stamps = f->stamps;
+  }
   P.late_share = f->share && f->sel == 2 ? (int)f->late_share : 0;  // a form of splitting
   P.share_work = (int)std::min<int64_t>(f->share_work, INT32_MAX);
   P.share_queue = f->share_queue >= 0 ? (int)f->share_queue : f->nwaves;
@@ -659,6 +681,7 @@ int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res) {
                        (unsigned long long)f->qcap);
     P.s.root_tasks = f->deferred;
     P.s.n_roots = f->last_deferred;
+    P.exit_stamps = nullptr;  // the stamps describe the main pass
     P.s.deferred = nullptr;
     P.s.deferred_cap = 0;
     KernelChoice wk = pick_kernel(8, 4, true);
@@ -689,6 +712,15 @@ int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res) {
   f->list_used = std::min<uint64_t>(ctl_out[C_LIST_CURSOR * kCtlStride], f->list_words);
   f->list_ovf = acc_h[mbe::A_LIST_OVF];
   memcpy(f->prof, acc_h, sizeof(acc_h));
+  f->exit_s.clear();
+  if (P.exit_stamps && n_roots > 0) {
+    std::vector<unsigned long long> st(2ull * f->nblocks);
+    HIP_TRY(hipMemcpy(st.data(), f->stamps, st.size() * 8, hipMemcpyDeviceToHost));
+    unsigned long long t0s = ~0ull;
+    for (int b = 0; b < f->nblocks; ++b) t0s = std::min<unsigned long long>(t0s, st[2 * b]);
+    for (int b = 0; b < f->nblocks; ++b)
+      f->exit_s.push_back((double)(st[2 * b + 1] - t0s) / ((double)f->eng.wall_khz * 1e3));
+  }
   const uint64_t roots_claimed = std::min<uint64_t>(ctl_out[C_ROOT_NEXT * kCtlStride], n_roots);
   if (f->deadline_ms > 0 &&
       (acc_h[mbe::A_ABANDONED] > 0 || ctl_out[C_WORK_LEFT * kCtlStride] != 0 ||
@@ -705,6 +737,14 @@ int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res) {
                      (unsigned long long)res->overflow, (unsigned long long)acc_h[mbe::A_OVF_ARENA],
                      (long long)f->arena_kb, (unsigned long long)acc_h[mbe::A_OVF_WIDE],
                      mbe::kMaxW * 64, (unsigned long long)acc_h[mbe::A_OVF_DEFER]);
+  return GMBE_OK;
+}
+
+int gmbe_finder_exit_times(const gmbe_finder* f, double* out, size_t cap, size_t* n) {
+  if (!f || !n) return set_error(GMBE_ERR_ARG, "null argument");
+  if (f->exit_s.empty()) return set_error(GMBE_ERR_STATE, "no run with exit_stamps set");
+  *n = f->exit_s.size();
+  if (out) memcpy(out, f->exit_s.data(), std::min(cap, f->exit_s.size()) * sizeof(double));
   return GMBE_OK;
 }
 
@@ -761,6 +801,7 @@ void gmbe_finder_destroy(gmbe_finder* f) {
   (void)hipFree(f->ctl);
   (void)hipFree(f->acc);
   (void)hipFree(f->gt);
+  (void)hipFree(f->stamps);
   (void)hipFree(f->list);
   (void)hipFree(f->deferred);
   if (f->ev0) (void)hipEventDestroy(f->ev0);

@@ -15,6 +15,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <vector>
 #include <cstdlib>
 #include <cstring>
 
@@ -113,6 +114,13 @@ class BicliqueFinder {
 // The hot path: IterFinderGpu6 (src/IterFinderGpu.h:77-90).  sel picks the
 // scheduler; the reference's IterFinderGpu (sel 0) and IterFinderGpu2 (sel 1)
 // map to the same kernel without task splitting.
+// -p of the reference's main (src/main.cu:121-123): print each block's exit
+// time after a run ("SM exit", src/IterFinderGpu.cu:1038-1041).
+inline bool& print_exit_times() {
+  static bool on = false;
+  return on;
+}
+
 class IterFinderGpu6 : public BicliqueFinder {
  public:
   IterFinderGpu6() = delete;
@@ -125,8 +133,17 @@ class IterFinderGpu6 : public BicliqueFinder {
   // src/IterFinderGpu.cu:1095-1119, same stdout lines
   void Execute() override {
     printf("large worklist size: %d local tiny size: %d\n", 1 << 22, 0);
+    if (print_exit_times()) gmbe_finder_set(finder_, "exit_stamps", 1);
     gmbe_result r;
     gmbe_assert(gmbe_finder_execute(finder_, &r), "IterFinderGpu6::Execute");
+    if (print_exit_times()) {
+      size_t n = 0;
+      if (gmbe_finder_exit_times(finder_, nullptr, 0, &n) == 0 && n > 0) {
+        std::vector<double> t(n);
+        gmbe_finder_exit_times(finder_, t.data(), n, &n);
+        for (double x : t) printf("SM exit: %lf\n", x);
+      }
+    }
     printf("%d\n", 0);
     printf("non_maximal: %lld\n", (long long)r.non_maximal);
     maximal_nodes_ = r.maximal;

@@ -48,7 +48,7 @@ int main(int argc, char* argv[]) {
       case 'm': bound_height = atoi(optarg); break;
       case 'n': bound_size = atoi(optarg); break;
       case 'x': ngpus = atoi(optarg); break;
-      case 'p': break;  // per-CU exit stamps are reported by the profiler instead
+      case 'p': print_exit_times() = true; break;  // per-block exit times after the run
       default: break;
     }
   }

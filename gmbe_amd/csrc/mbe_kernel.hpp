@@ -137,6 +137,7 @@ struct Params {
   int late_share;            // min list length of a frame handed over late (0 = off)
   uint64_t budget;           // wall-clock ticks a wave may run (0 = no deadline)
   uint64_t deadline;         // set per wave: its start + budget
+  unsigned long long* exit_stamps;  // if set: wall clock at which each block's waves were done
 };
 
 struct Counters {

@@ -49,6 +49,14 @@ class IterFinderGpu6:
         check(lib().gmbe_finder_list(self._h, buf.ctypes.data, n, C.byref(used)))
         return records_to_set(buf[:n])
 
+    def exit_times(self) -> list:
+        """Per-block exit times (s) of the last run with exit_stamps=1."""
+        n = C.c_size_t()
+        check(lib().gmbe_finder_exit_times(self._h, None, 0, C.byref(n)))
+        buf = (C.c_double * max(n.value, 1))()
+        check(lib().gmbe_finder_exit_times(self._h, buf, n.value, C.byref(n)))
+        return [buf[i] for i in range(n.value)]
+
     STAT_NAMES = ["cyc_wait", "cyc_prefix", "cyc_expand", "cyc_extract", "cyc_build", "cyc_dfs",
                   "cyc_split", "edges", "entries", "ovf_arena", "ovf_wide", "ovf_defer",
                   "entries_w1", "nodes_w1", "cyc_tail", "sub_pass", "sub_merge",

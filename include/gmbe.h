@@ -209,6 +209,12 @@ int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res);
  * GMBE_ERR_CAPACITY if cap is too small or the device buffer overflowed. */
 int gmbe_finder_list(gmbe_finder* f, uint32_t* buf, size_t cap, size_t* used);
 
+/* Per-block exit times of the last execute with the knob "exit_stamps" = 1:
+ * seconds from the earliest block start to each block's end (the -p output
+ * of the reference, "SM exit" per SM, src/IterFinderGpu.cu:1038-1041).
+ * *n receives the block count; up to cap values are copied into out. */
+int gmbe_finder_exit_times(const gmbe_finder* f, double* out, size_t cap, size_t* n);
+
 /* Profile of the last execute (no reference counterpart; the reference's
  * -p prints per-SM exit times, src/IterFinderGpu.cu:1038-1041).  Copies up to n
  * counters: [0] waiting for work, [1] prefix intersections, [2] 2-hop edge

@@ -223,3 +223,16 @@ def test_unsplit_matches_reference_traversal(gpu):
         assert r.nodes == o.nodes
         assert r.non_maximal == o.non_maximal
         assert r.alg_bytes == o.b_alg
+
+
+def test_exit_times_of_the_blocks(gpu):
+    """exit_stamps=1 (the CLI's -p): one exit time per block of the persistent
+    grid, none negative or past the run, and the result unchanged."""
+    g = workloads.run_graph(1)
+    f = IterFinderGpu6(g, exit_stamps=1)
+    r = f.execute()
+    t = f.exit_times()
+    f.close()
+    assert len(t) > 0
+    assert all(0.0 <= x <= r.exe_time_s + 0.1 for x in t)
+    assert r.digest() == gpu_digest(g).digest()
