@@ -215,8 +215,8 @@ int gmbe_finder_list(gmbe_finder* f, uint32_t* buf, size_t cap, size_t* used);
  * *n receives the block count; up to cap values are copied into out. */
 int gmbe_finder_exit_times(const gmbe_finder* f, double* out, size_t cap, size_t* n);
 
-/* Profile of the last execute (no reference counterpart; the reference's
- * -p prints per-SM exit times, src/IterFinderGpu.cu:1038-1041).  Copies up to n
+/* Profile of the last execute (no reference counterpart; per-block exit
+ * times are gmbe_finder_exit_times above).  Copies up to n
  * counters: [0] waiting for work, [1] prefix intersections, [2] 2-hop edge
  * pass, [3] extraction + canonicity, [4] candidate-list build, [5] stack
  * machine, [6] split pushes (cycles summed over waves, filled when the knob
@@ -226,7 +226,8 @@ int gmbe_finder_exit_times(const gmbe_finder* f, double* out, size_t cap, size_t
  * [14] cycles of the final wait (a wave with nothing left to do);
  * [15..18] cycles inside the stack machine spent in push passes, twin
  * merges, bitset re-indexing and context publishing; [19] contexts
- * published, [20] twin merges.
+ * published, [20] twin merges; [21] subtrees abandoned at the deadline;
+ * [22] frames handed over by late sharing.
  * Returns the number of counters available. */
 int gmbe_finder_stats(gmbe_finder* f, uint64_t* out, size_t n);
 
