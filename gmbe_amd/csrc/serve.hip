@@ -70,7 +70,7 @@ struct gmbe_server {
   std::deque<gmbe_serve_done> done;
   uint64_t next_ticket = 1;
   int64_t request_ms = 0;  // deadline of each launch (0 = none)
-  // statistics (server thread)
+  // statistics (written by the server thread under mu)
   uint64_t launches = 0, launched = 0;
   double host_s = 0, gpu_s = 0;
   uint64_t outstanding = 0;
@@ -85,7 +85,7 @@ void finish_lane(gmbe_server* s, int l) {
   Lane& L = s->lanes[l];
   const double t = now_s();
   float ms = 0;
-  if (hipEventElapsedTime(&ms, L.ev0, L.ev1) == hipSuccess) s->gpu_s += ms * 1e-3;
+  if (hipEventElapsedTime(&ms, L.ev0, L.ev1) != hipSuccess) ms = 0;
   std::vector<gmbe_serve_done> out;
   for (size_t i = 0; i < s->in_lane[l].size(); ++i) {
     Request& r = s->in_lane[l][i];
@@ -105,6 +105,7 @@ void finish_lane(gmbe_server* s, int l) {
   s->in_lane[l].clear();
   s->busy[l] = false;
   std::lock_guard<std::mutex> lk(s->mu);
+  s->gpu_s += ms * 1e-3;
   for (auto& d : out) s->done.push_back(d);
   s->outstanding -= out.size();
   s->cv_done.notify_all();
@@ -123,9 +124,12 @@ int launch_lane(gmbe_server* s, int l, std::vector<Request>& batch) {
   int rc = L.upload_many(parts, L.knobs.budget_ms > 0);
   if (rc) return rc;
   if (hipEventRecord(L.ev0, L.st) != hipSuccess) return set_error(GMBE_ERR_HIP, "event record");
-  s->host_s += now_s() - t0;
-  s->launches++;
-  s->launched += batch.size();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);  // the statistics are read under the lock
+    s->host_s += now_s() - t0;
+    s->launches++;
+    s->launched += batch.size();
+  }
   rc = L.record_run();
   if (rc) return rc;
   if (hipEventRecord(L.ev1, L.st) != hipSuccess) return set_error(GMBE_ERR_HIP, "event record");
