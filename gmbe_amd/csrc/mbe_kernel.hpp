@@ -40,7 +40,6 @@
 namespace mbe {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
-constexpr uint32_t kBranch = 0x80000000u;  // entry may still be pushed
 constexpr int kMaxPrefix = 8;              // task capacity (vertices)
 constexpr int kShards = 32;                // independent queue rings
 constexpr int kCtlStride = 32;             // u64 between control counters (256 B)
