@@ -169,3 +169,28 @@ def test_root_sample_partition():
     for p in parts:
         x ^= p.hash_xor
     assert x == full.hash_xor
+
+
+def test_mbea_time_limit_cuts_within_a_root():
+    """orc_mbea's time limit also abandons the subtree it is in: a tiny limit
+    returns promptly with a partial count, a generous one the full count."""
+    og = workloads.run_graph(2, 0.01).oracle_graph()
+    full = oracle.mbea(og, threads=8)
+    cut = oracle.mbea(og, threads=8, time_limit=0.01)
+    assert cut.seconds < 2.0
+    assert cut.maximal < full.maximal
+    again = oracle.mbea(og, threads=8, time_limit=600.0)
+    assert again.digest() == full.digest()
+
+
+def test_rank_offset_samples_partition_the_roots():
+    """Samples that start at a later rank (phase >= stride) select the ranks
+    phase, phase + stride, ...: two offsets of one stride add up to the
+    stride's sample from the earlier rank."""
+    og = workloads.run_graph(2, 0.01).oracle_graph()
+    a = oracle.gmbe(og, threads=4, stride=2, phase=10)   # ranks 10, 12, 14, ...
+    b = oracle.gmbe(og, threads=4, stride=2, phase=11)   # ranks 11, 13, 15, ...
+    c = oracle.gmbe(og, threads=4, stride=1, phase=10)   # ranks 10, 11, 12, ...
+    assert a.maximal + b.maximal == c.maximal
+    assert (a.hash_sum + b.hash_sum) % (1 << 64) == c.hash_sum
+    assert a.hash_xor ^ b.hash_xor == c.hash_xor
