@@ -50,16 +50,20 @@ def workload_name(config: int, scale: float, root_begin: int) -> str:
             f"top-level vertices of processing rank >= {root_begin}")
 
 
-def cpu_baseline(graph, root_begin: int, seconds: float):
-    """The oracle's MbeaFinder restatement on a sample of the step's roots."""
+def cpu_baseline(graph, root_begin: int, seconds: float, algo: str = "mbea"):
+    """The oracle's MbeaFinder restatement (or, with algo="imbea", the
+    authors' improved ImbeaAdvFinder) on a sample of the step's roots."""
     import oracle
+    run = oracle.mbea if algo == "mbea" else oracle.imbea
+    name = ("MbeaFinder restatement (oracle/oracle.c orc_mbea" if algo == "mbea"
+            else "ImbeaAdvFinder restatement (oracle/oracle.c orc_imbea")
     threads = min(16, os.cpu_count() or 1)
     stride = 97
     og = graph.oracle_graph()
     maximal, secs, tasks, phases = 0, 0.0, 0, 0
     for j in range(stride):
-        r = oracle.mbea(og, threads=threads, stride=stride, phase=root_begin + j,
-                        time_limit=max(1.0, seconds - secs))
+        r = run(og, threads=threads, stride=stride, phase=root_begin + j,
+                time_limit=max(1.0, seconds - secs))
         maximal += r.maximal
         secs += r.seconds
         tasks += r.tasks
@@ -69,7 +73,7 @@ def cpu_baseline(graph, root_begin: int, seconds: float):
     return {
         "value": maximal / secs if secs > 0 else 0.0, "unit": "bicliques/s", "cores": threads,
         "kind": "port",
-        "sample": (f"MbeaFinder restatement (oracle/oracle.c orc_mbea, {threads} threads) over "
+        "sample": (f"{name}, {threads} threads) over "
                    f"{tasks} top-level vertices of the same step (ranks {root_begin}+j+97t, "
                    f"j < {phases}): {maximal} maximal bicliques in {secs:.1f} s"),
     }
@@ -427,6 +431,9 @@ def main():
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(graph, args.root_begin, args.cpu_seconds)
+        # the authors' improved CPU finder on the same sample, for comparison
+        line["cpu_baseline_imbea"] = cpu_baseline(graph, args.root_begin, args.cpu_seconds / 3,
+                                                  algo="imbea")
     print(json.dumps(line), flush=True)
 
 

@@ -50,6 +50,8 @@ def _l():
         l.orc_mbea.restype = C.c_int
         l.orc_mbea.argtypes = graph + [C.c_int, C.c_uint32, C.c_uint32, C.c_double,
                                        C.POINTER(OracleResult), P]
+        l.orc_imbea.restype = C.c_int
+        l.orc_imbea.argtypes = l.orc_mbea.argtypes
         l.orc_brute.restype = C.c_int
         l.orc_brute.argtypes = graph + [C.POINTER(OracleResult), P]
         l.orc_list_new.restype = P
@@ -112,6 +114,21 @@ def mbea(g, threads=1, stride=1, phase=0, time_limit=0.0, want_list=False):
     lst = _take_list(h) if want_list else None
     if rc != 0:
         raise RuntimeError(f"orc_mbea failed: {rc}")
+    del keep
+    return (res, lst) if want_list else res
+
+
+def imbea(g, threads=1, stride=1, phase=0, time_limit=0.0, want_list=False):
+    """iMBEA restatement (ImbeaAdvFinder of baselines/MBE.zip): the authors'
+    improved CPU finder, a further CPU baseline."""
+    l = _l()
+    args, keep = _graph_args(g)
+    res = OracleResult()
+    h = l.orc_list_new() if want_list else None
+    rc = l.orc_imbea(*args, threads, stride, phase, time_limit, C.byref(res), h)
+    lst = _take_list(h) if want_list else None
+    if rc != 0:
+        raise RuntimeError(f"orc_imbea failed: {rc}")
     del keep
     return (res, lst) if want_list else res
 

@@ -20,6 +20,10 @@
  *              (src/BicliqueFinder.cu:18-57) with the 2-hop top level of
  *              MbeaAdvFinder (baselines/MBE.zip:MBE/BicliqueFinder.cpp:42-88),
  *              multi-threaded over top-level vertices.
+ *   orc_imbea  restates the authors' improved CPU finder ImbeaAdvFinder
+ *              (baselines/MBE.zip:MBE/BicliqueFinder.cpp:227-350): candidates
+ *              ordered by overlap, dominated candidates moved to the excluded
+ *              set with the pushed vertex; a further CPU baseline.
  *   orc_brute  checks every subset R of V (|V| <= 24, |U| <= 64): the
  *              definition of a maximal biclique, L = N(R) != {} and
  *              R = N(L).
@@ -717,10 +721,195 @@ static int mb_root(mb_ws* w, uint32_t v) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* iMBEA restatement (the authors' improved CPU finder, -s 1 of MBE.zip)     */
+
+typedef struct {
+  uint32_t id;
+  int64_t cnt;
+} im_pair;
+
+static int im_by_count_desc(const void* a, const void* b) {
+  const im_pair* x = (const im_pair*)a;
+  const im_pair* y = (const im_pair*)b;
+  if (x->cnt != y->cnt) return x->cnt > y->cnt ? -1 : 1;
+  return x->id > y->id ? -1 : (x->id < y->id ? 1 : 0);
+}
+
+/* n pairs from the u32 arena, 8-byte aligned */
+static im_pair* im_pairs(mb_ws* w, size_t n) {
+  uint32_t* p = mb_alloc(w, 4 * (n + 1) + 2);
+  if (!p) return NULL;
+  uintptr_t a = ((uintptr_t)p + 7u) & ~(uintptr_t)7u;
+  return (im_pair*)a;
+}
+
+/* a \ b for ascending lists */
+static int64_t except(const uint32_t* a, int64_t na, const uint32_t* b, int64_t nb, uint32_t* out) {
+  int64_t i = 0, j = 0, n = 0;
+  while (i < na) {
+    while (j < nb && b[j] < a[i]) ++j;
+    if (j < nb && b[j] == a[i]) {
+      ++i;
+      ++j;
+    } else {
+      out[n++] = a[i++];
+    }
+  }
+  return n;
+}
+
+/* ImbeaAdvFinder::biclique_find (baselines/MBE.zip:MBE/BicliqueFinder.cpp:
+ * 282-350).  P is processed from its back; a candidate whose neighbourhood
+ * holds L' and nothing of L \ L' is moved to Q with x after x's subtree.  P
+ * and Q are this call's own arrays; Q has room for every entry of P. */
+static int im_find(mb_ws* w, const uint32_t* L, int64_t nL, const uint32_t* R, int64_t nR,
+                   uint64_t sR, uint32_t* P, int64_t nP, uint32_t* Q, int64_t nQ) {
+  const og* g = w->g;
+  while (nP > 0) {
+    if (w->deadline > 0 && (++w->tick & 255u) == 0 && wall() > w->deadline) w->cut = 1;
+    if (w->cut) return 0;
+    size_t mark = w->arena_n;
+    uint32_t x = P[--nP];
+    uint32_t* Lp = mb_alloc(w, (size_t)nL + 1);
+    uint32_t* Lc = mb_alloc(w, (size_t)nL + 1);
+    if (!Lp || !Lc) return -1;
+    int64_t nLp = intersect(L, nL, NV_ROW(g, x), NV_DEG(g, x), Lp);
+    int64_t nLc = except(L, nL, Lp, nLp, Lc);
+    if (nLp == 0) {
+      Q[nQ++] = x;
+      w->arena_n = mark;
+      continue;
+    }
+    w->res.nodes++;
+    uint32_t* Qp = mb_alloc(w, (size_t)(nQ + nP) + 1);
+    if (!Qp) return -1;
+    int64_t nQp = 0;
+    int maximal = 1;
+    for (int64_t i = 0; i < nQ && maximal; ++i) {
+      int64_t nc = intersect_cnt(Lp, nLp, NV_ROW(g, Q[i]), NV_DEG(g, Q[i]));
+      if (nc == nLp) maximal = 0;
+      else if (nc > 0) Qp[nQp++] = Q[i];
+    }
+    if (!maximal) {
+      w->res.non_maximal++;
+      w->arena_n = mark;
+      continue;
+    }
+    uint32_t* Rp = mb_alloc(w, (size_t)(nR + nP) + 2);
+    uint32_t* Cx = mb_alloc(w, (size_t)nP + 2);
+    im_pair* pp = im_pairs(w, (size_t)nP);
+    if (!Rp || !Cx || !pp) return -1;
+    memcpy(Rp, R, (size_t)nR * 4);
+    int64_t nRp = nR, nCx = 0, npp = 0;
+    Rp[nRp++] = x;
+    uint64_t sRp = sR + g->kV[x];
+    Cx[nCx++] = x;
+    for (int64_t i = 0; i < nP; ++i) {
+      const uint32_t p = P[i];
+      int64_t nc = intersect_cnt(Lp, nLp, NV_ROW(g, p), NV_DEG(g, p));
+      if (nc == nLp) {
+        Rp[nRp++] = p;
+        sRp += g->kV[p];
+        if (intersect_cnt(Lc, nLc, NV_ROW(g, p), NV_DEG(g, p)) == 0) Cx[nCx++] = p;
+      } else if (nc > 0) {
+        pp[npp].id = p;
+        pp[npp].cnt = nc;
+        ++npp;
+      }
+    }
+    if (npp > 0) {
+      qsort(pp, (size_t)npp, sizeof(im_pair), im_by_count_desc);
+      uint32_t* Pp = mb_alloc(w, (size_t)npp + 1);
+      if (!Pp) return -1;
+      for (int64_t i = 0; i < npp; ++i) Pp[i] = pp[i].id;
+      if (im_find(w, Lp, nLp, Rp, nRp, sRp, Pp, npp, Qp, nQp) < 0) return -1;
+    }
+    uint64_t sL = 0;
+    for (int64_t j = 0; j < nLp; ++j) sL += g->kU[Lp[j]];
+    add_biclique(&w->res, hash_internal(g, sL, sRp));
+    if (w->list) {
+      qsort(Rp, (size_t)nRp, 4, cmp_u32);
+      record(g, w->list, Lp, (size_t)nLp, Rp, (size_t)nRp);
+    }
+    /* P = P \ C (order kept), Q += C */
+    int64_t k = 0;
+    for (int64_t i = 0; i < nP; ++i) {
+      int in_c = 0;
+      for (int64_t c = 1; c < nCx && !in_c; ++c) in_c = Cx[c] == P[i];
+      if (!in_c) P[k++] = P[i];
+    }
+    nP = k;
+    for (int64_t c = 0; c < nCx; ++c) Q[nQ++] = Cx[c];
+    w->arena_n = mark;
+  }
+  return 0;
+}
+
+/* ImbeaAdvFinder::Execute, one top-level vertex (baselines/MBE.zip:MBE/
+ * BicliqueFinder.cpp:227-280): 2-hop counts, canonicity against lower ids,
+ * the root biclique, then the search over the higher 2-hop neighbours
+ * ordered by decreasing overlap. */
+static int im_root(mb_ws* w, uint32_t v) {
+  const og* g = w->g;
+  const uint32_t* L = NV_ROW(g, v);
+  int64_t nL = NV_DEG(g, v);
+  w->res.tasks++;
+  w->res.nodes++;
+  if (nL == 0) return 0;
+  int64_t nt = 0;
+  for (int64_t j = 0; j < nL; ++j) {
+    const uint32_t* row = NU_ROW(g, L[j]);
+    int64_t d = NU_DEG(g, L[j]);
+    for (int64_t e = 0; e < d; ++e)
+      if (w->cmap[row[e]]++ == 0) w->touched[nt++] = row[e];
+  }
+  int canonical = 1;
+  for (int64_t i = 0; i < nt; ++i)
+    if (w->cmap[w->touched[i]] == nL && w->touched[i] < v) canonical = 0;
+  if (!canonical) {
+    w->res.non_maximal++;
+    for (int64_t i = 0; i < nt; ++i) w->cmap[w->touched[i]] = 0;
+    return 0;
+  }
+  size_t mark = w->arena_n;
+  uint32_t* R = mb_alloc(w, (size_t)nt + 1);
+  uint32_t* Q = mb_alloc(w, (size_t)nt + 1);
+  uint32_t* P = mb_alloc(w, (size_t)nt + 1);
+  im_pair* pp = im_pairs(w, (size_t)nt);
+  if (!R || !Q || !P || !pp) return -1;
+  int64_t nR = 0, nQ = 0, npp = 0;
+  uint64_t sR = 0, sL = 0;
+  for (int64_t i = 0; i < nt; ++i) {
+    uint32_t y = w->touched[i];
+    int64_t c = w->cmap[y];
+    w->cmap[y] = 0;
+    if (c == nL) {
+      R[nR++] = y;
+      sR += g->kV[y];
+    } else if (y < v) {
+      Q[nQ++] = y;
+    } else {
+      pp[npp].id = y;
+      pp[npp].cnt = c;
+      ++npp;
+    }
+  }
+  qsort(R, (size_t)nR, 4, cmp_u32);
+  for (int64_t j = 0; j < nL; ++j) sL += g->kU[L[j]];
+  add_biclique(&w->res, hash_internal(g, sL, sR));
+  if (w->list) record(g, w->list, L, (size_t)nL, R, (size_t)nR);
+  qsort(pp, (size_t)npp, sizeof(im_pair), im_by_count_desc);
+  for (int64_t i = 0; i < npp; ++i) P[i] = pp[i].id;
+  int rc = im_find(w, L, nL, R, nR, sR, P, npp, Q, nQ);
+  w->arena_n = mark;
+  return rc;
+}
+
+/* ------------------------------------------------------------------------ */
 /* threading                                                                 */
 
 typedef struct {
-  int kind; /* 0 gmbe, 1 mbea */
+  int kind; /* 0 gmbe, 1 mbea, 2 imbea */
   const og* g;
   gm_params gp;
   int want_list;
@@ -785,7 +974,7 @@ static void* run_worker(void* arg) {
         if (t >= j->n_roots || j->failed) break;
         if (j->deadline > 0 && wall() > j->deadline) break;
         uint32_t v = (uint32_t)(g->nV - 1 - ((uint64_t)t * j->stride + j->phase));
-        if (mb_root(&w, v) < 0) {
+        if ((j->kind == 2 ? im_root(&w, v) : mb_root(&w, v)) < 0) {
           wk->rc = -2;
           j->failed = 1;
           break;
@@ -871,6 +1060,15 @@ int orc_gmbe(uint32_t nU, uint32_t nV, const int64_t* v_off, const uint32_t* v_a
              uint32_t phase, double time_limit, orc_result* out, orc_list* list) {
   gm_params p = {bound_height, bound_size, prune, max_prefix < 1 ? 4 : (max_prefix > 15 ? 15 : max_prefix)};
   return run(0, nU, nV, v_off, v_adj, u_orig, v_orig, swapped, p, nthreads, stride, phase,
+             time_limit, out, list);
+}
+
+int orc_imbea(uint32_t nU, uint32_t nV, const int64_t* v_off, const uint32_t* v_adj,
+              const uint32_t* u_orig, const uint32_t* v_orig, int swapped, int nthreads,
+              uint32_t stride, uint32_t phase, double time_limit, orc_result* out,
+              orc_list* list) {
+  gm_params p = {-1, 0, 1, 4};
+  return run(2, nU, nV, v_off, v_adj, u_orig, v_orig, swapped, p, nthreads, stride, phase,
              time_limit, out, list);
 }
 

@@ -91,6 +91,7 @@ def test_oracle_matches_reference_cpu_counts(name):
     assert oracle.gmbe(og, threads=8).maximal == ent["count"]
     if ent["count"] < 5_000_000:
         assert oracle.mbea(og, threads=8).maximal == ent["count"]
+        assert oracle.imbea(og, threads=8).maximal == ent["count"]
 
 
 @pytest.mark.parametrize("name", sorted(graphs.KATS))
@@ -126,10 +127,12 @@ def test_three_way_agreement_small(seed):
     rb, lb = oracle.brute(og, want_list=True)
     rg, lg = oracle.gmbe(og, want_list=True)
     rm, lm = oracle.mbea(og, want_list=True)
+    ri, li = oracle.imbea(og, want_list=True)
     sb = records_to_set(lb)
     assert records_to_set(lg) == sb
     assert records_to_set(lm) == sb
-    assert rb.digest() == rg.digest() == rm.digest() == digest_of(sb)
+    assert records_to_set(li) == sb
+    assert rb.digest() == rg.digest() == rm.digest() == ri.digest() == digest_of(sb)
 
 
 @pytest.mark.parametrize("opt", [(0, 0), (1, 1), (2, 2), (1, 0), (0, 3)])
