@@ -28,3 +28,20 @@ def test_loader_and_generators_are_sanitizer_clean(tmp_path):
                        env=env)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "0 failures" in r.stdout
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="no host C compiler")
+def test_oracle_is_sanitizer_clean(tmp_path):
+    """The CPU oracle itself: the four enumerators agree on random small
+    graphs, lists on, worker threads on, under ASan and UBSan."""
+    exe = tmp_path / "drive_oracle"
+    cmd = ["gcc", "-std=c11", "-g", "-O1", "-fsanitize=address,undefined",
+           "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer", "-pthread",
+           "-o", str(exe), os.path.join(HERE, "host", "sanitize_oracle.c"),
+           os.path.join(ROOT, "oracle", "oracle.c"), "-lm"]
+    subprocess.run(cmd, check=True, timeout=300)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:verify_asan_link_order=0",
+               UBSAN_OPTIONS="halt_on_error=1")
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "0 failures" in r.stdout
