@@ -499,7 +499,7 @@ int gmbe_finder_create(const gmbe_graph* g, int sel, int bound_height, int bound
   if (sel < 0 || sel > 3) return set_error(GMBE_ERR_ARG, "sel must be 0..3 (got %d)", sel);
   if (ngpus != 1 || sel == 3)
     return set_error(GMBE_ERR_UNSUPPORTED,
-                     "this build runs on one GPU (sel=%d ngpus=%d); see the sharding design",
+                     "this engine runs on one GPU (sel=%d ngpus=%d)",
                      sel, ngpus);
   if (bound_height < 0 || bound_size < 0)
     return set_error(GMBE_ERR_ARG, "bounds must be non-negative");
