@@ -3,5 +3,5 @@
 Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this
 package, as the checker; the engine (gmbe_amd) never imports it.
 """
-from .binding import (OracleResult, available, brute, gmbe, imbea, mbea, hash_pair,  # noqa: F401
-                      LIB_PATH)
+from .binding import (OracleResult, available, brute, gmbe, imbea, mbea, pmbe,  # noqa: F401
+                      hash_pair, LIB_PATH)

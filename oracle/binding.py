@@ -21,7 +21,7 @@ class OracleResult(C.Structure):
         ("maximal", C.c_uint64), ("non_maximal", C.c_uint64),
         ("hash_sum", C.c_uint64), ("hash_xor", C.c_uint64),
         ("nodes", C.c_uint64), ("b_alg", C.c_uint64), ("tasks", C.c_uint64),
-        ("splits", C.c_uint64), ("seconds", C.c_double),
+        ("splits", C.c_uint64), ("skipped", C.c_uint64), ("seconds", C.c_double),
     ]
 
     def digest(self):
@@ -52,6 +52,8 @@ def _l():
                                        C.POINTER(OracleResult), P]
         l.orc_imbea.restype = C.c_int
         l.orc_imbea.argtypes = l.orc_mbea.argtypes
+        l.orc_pmbe.restype = C.c_int
+        l.orc_pmbe.argtypes = l.orc_mbea.argtypes
         l.orc_brute.restype = C.c_int
         l.orc_brute.argtypes = graph + [C.POINTER(OracleResult), P]
         l.orc_list_new.restype = P
@@ -129,6 +131,22 @@ def imbea(g, threads=1, stride=1, phase=0, time_limit=0.0, want_list=False):
     lst = _take_list(h) if want_list else None
     if rc != 0:
         raise RuntimeError(f"orc_imbea failed: {rc}")
+    del keep
+    return (res, lst) if want_list else res
+
+
+def pmbe(g, threads=1, stride=1, phase=0, time_limit=0.0, want_list=False):
+    """PMBE restatement (PmbeAdvFinder of baselines/MBE.zip): the authors' CPU
+    finder over the neighbourhood-containment order, a further CPU baseline.
+    Processing ranks follow that order, not the other finders' ids."""
+    l = _l()
+    args, keep = _graph_args(g)
+    res = OracleResult()
+    h = l.orc_list_new() if want_list else None
+    rc = l.orc_pmbe(*args, threads, stride, phase, time_limit, C.byref(res), h)
+    lst = _take_list(h) if want_list else None
+    if rc != 0:
+        raise RuntimeError(f"orc_pmbe failed: {rc}")
     del keep
     return (res, lst) if want_list else res
 

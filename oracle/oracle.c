@@ -4,7 +4,7 @@
  * TEST INFRASTRUCTURE ONLY.  This file is the checker: tests, the smoke check
  * and the CPU-baseline leg of the benchmark call it; the engine never does.
  *
- * Three independent enumerators over the same CSR input:
+ * Independent enumerators over the same CSR input:
  *
  *   orc_gmbe   restates the reference's GPU traversal in its own order:
  *              task roots (IterFinderWithMultipleVertex,
@@ -24,6 +24,11 @@
  *              (baselines/MBE.zip:MBE/BicliqueFinder.cpp:227-350): candidates
  *              ordered by overlap, dominated candidates moved to the excluded
  *              set with the pushed vertex; a further CPU baseline.
+ *   orc_pmbe   restates the authors' CPU finder PmbeAdvFinder
+ *              (baselines/MBE.zip:MBE/PMBE.cpp:3-77, 251-354): V renumbered
+ *              by the neighbourhood-containment order, candidates skipped
+ *              inside the range a done candidate released, maximality by
+ *              N(L') \ R within the candidates; a further CPU baseline.
  *   orc_brute  checks every subset R of V (|V| <= 24, |U| <= 64): the
  *              definition of a maximal biclique, L = N(R) != {} and
  *              R = N(L).
@@ -60,7 +65,9 @@
 #define LV_TOUCHED 0x3fffffff
 
 typedef struct {
-  uint64_t maximal, non_maximal, hash_sum, hash_xor, nodes, b_alg, tasks, roots;
+  uint64_t maximal, non_maximal, hash_sum, hash_xor, nodes, b_alg, tasks;
+  uint64_t splits;  /* orc_gmbe: tasks split by the bound rule */
+  uint64_t skipped; /* orc_pmbe: candidates skipped inside a containment range */
   double seconds;
 } orc_result;
 
@@ -576,7 +583,7 @@ static void gm_task(gm_ws* w, const uint32_t* S, int k) {
   if (w->p.bound_height >= 0 && k < w->p.max_prefix &&
       (rest > w->p.bound_height || nL > w->p.bound_height) &&
       rest * nL > (int64_t)w->p.bound_size) {
-    w->res.roots++; /* number of splits */
+    w->res.splits++;
     uint32_t* kids = (uint32_t*)malloc((size_t)(rest ? rest : 1) * 4);
     memcpy(kids, w->C + next_id, (size_t)rest * 4);
     uint32_t S2[16];
@@ -609,6 +616,7 @@ typedef struct {
   double deadline; /* 0 = none; past it, subtrees are abandoned (counts partial) */
   uint64_t tick;
   int cut;
+  const int32_t* range; /* PMBE: released-children range per V vertex, 2 ints */
 } mb_ws;
 
 static uint32_t* mb_alloc(mb_ws* w, size_t n) {
@@ -906,11 +914,283 @@ static int im_root(mb_ws* w, uint32_t v) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* PMBE restatement (the authors' CPU finder with the containment order,     */
+/* -s 2 of MBE.zip)                                                          */
+
+typedef struct {
+  uint32_t deg, id;
+} pm_key;
+
+/* released vertices, larger neighbourhood first, then ascending id.  The
+ * reference sorts them with std::sort by degree only, which leaves the order
+ * of equal degrees unspecified; any order is a valid topological order and
+ * yields the same bicliques. */
+static int pm_by_deg_desc(const void* a, const void* b) {
+  const pm_key* x = (const pm_key*)a;
+  const pm_key* y = (const pm_key*)b;
+  if (x->deg != y->deg) return x->deg > y->deg ? -1 : 1;
+  return x->id < y->id ? -1 : (x->id > y->id ? 1 : 0);
+}
+
+/* PmbeBiGraph's vertex order (baselines/MBE.zip:MBE/PMBE.cpp:3-77).  y is a
+ * parent of x when N(x) ⊆ N(y); of two equal neighbourhoods the lower id is
+ * the parent (an empty one is the child of every non-empty one).  Kahn's
+ * algorithm over that DAG hands out new ids from nV - 1 downwards; the
+ * vertices one vertex releases get consecutive ids below those still queued,
+ * and range[2 * new id] .. range[2 * new id + 1] holds them (first > second:
+ * none).  rename[new id] = old id.  Containment is found by 2-hop counting
+ * instead of the reference's all-pairs intersections; the DAG is the same. */
+static int pm_order(uint32_t nU, uint32_t nV, const int64_t* v_off, const uint32_t* v_adj,
+                    uint32_t* rename, int32_t* range) {
+  int64_t E = v_off[nV];
+  int64_t* u_off = calloc((size_t)nU + 1, sizeof(int64_t));
+  uint32_t* u_adj = malloc((size_t)(E ? E : 1) * 4);
+  int64_t* pos = malloc(((size_t)nU + 1) * sizeof(int64_t));
+  int32_t* cnt = calloc((size_t)nV + 1, 4);
+  uint32_t* touched = malloc(((size_t)nV + 1) * 4);
+  int64_t* indeg = calloc((size_t)nV + 1, sizeof(int64_t));
+  uint32_t* queue = malloc(((size_t)nV + 1) * 4);
+  pm_key* ready = malloc(((size_t)nV + 1) * sizeof(pm_key));
+  int rc = 0;
+  if (!u_off || !u_adj || !pos || !cnt || !touched || !indeg || !queue || !ready) {
+    rc = -1;
+    goto out;
+  }
+  for (int64_t e = 0; e < E; ++e) u_off[v_adj[e] + 1]++;
+  for (uint32_t u = 0; u < nU; ++u) u_off[u + 1] += u_off[u];
+  memcpy(pos, u_off, ((size_t)nU + 1) * sizeof(int64_t));
+  for (uint32_t v = 0; v < nV; ++v)
+    for (int64_t e = v_off[v]; e < v_off[v + 1]; ++e) u_adj[pos[v_adj[e]]++] = v;
+#define PM_DEG(v) (v_off[(v) + 1] - v_off[(v)])
+  /* 2-hop counts of x: cnt[y] = |N(x) ∩ N(y)| over the y that share a neighbour */
+#define PM_COUNT(x, nt)                                                  \
+  do {                                                                   \
+    nt = 0;                                                              \
+    for (int64_t e = v_off[(x)]; e < v_off[(x) + 1]; ++e) {              \
+      uint32_t u_ = v_adj[e];                                            \
+      for (int64_t f = u_off[u_]; f < u_off[u_ + 1]; ++f)                \
+        if (cnt[u_adj[f]]++ == 0) touched[nt++] = u_adj[f];              \
+    }                                                                    \
+  } while (0)
+  int64_t n_full = 0, n_empty = 0;
+  for (uint32_t x = 0; x < nV; ++x) {
+    if (PM_DEG(x) == 0) {
+      /* every non-empty vertex and every lower empty one is a parent */
+      indeg[x] = -1 - n_empty++; /* fixed up below */
+      continue;
+    }
+    ++n_full;
+    int64_t nt;
+    PM_COUNT(x, nt);
+    for (int64_t i = 0; i < nt; ++i) {
+      uint32_t y = touched[i];
+      if (y != x && cnt[y] == PM_DEG(x) && (PM_DEG(y) > PM_DEG(x) || y < x)) indeg[x]++;
+      cnt[y] = 0;
+    }
+  }
+  for (uint32_t x = 0; x < nV; ++x)
+    if (PM_DEG(x) == 0) indeg[x] = n_full + (-1 - indeg[x]);
+  int64_t head = 0, tail = 0, nr = 0, done_full = 0;
+  for (uint32_t x = 0; x < nV; ++x)
+    if (indeg[x] == 0) ready[nr++] = (pm_key){(uint32_t)PM_DEG(x), x};
+  qsort(ready, (size_t)nr, sizeof(pm_key), pm_by_deg_desc);
+  while (nr > 0) queue[tail++] = ready[--nr].id;
+  int64_t scan = nV;
+  while (head < tail) {
+    uint32_t r = queue[head++];
+    rename[--scan] = r;
+    int64_t start = scan - (tail - head), end = start - 1;
+    nr = 0;
+    if (PM_DEG(r) > 0) {
+      int64_t nt;
+      PM_COUNT(r, nt);
+      for (int64_t i = 0; i < nt; ++i) {
+        uint32_t y = touched[i];
+        if (y != r && cnt[y] == PM_DEG(y) && (PM_DEG(y) < PM_DEG(r) || y > r))
+          if (--indeg[y] == 0) ready[nr++] = (pm_key){(uint32_t)PM_DEG(y), y};
+        cnt[y] = 0;
+      }
+      /* a non-empty vertex is a parent of every empty one; the last of them
+       * releases the lowest empty id */
+      if (++done_full == n_full && n_empty > 0)
+        for (uint32_t x = 0; x < nV; ++x)
+          if (PM_DEG(x) == 0) {
+            ready[nr++] = (pm_key){0, x};
+            break;
+          }
+    } else {
+      /* an empty vertex releases the next empty id, its last parent */
+      for (uint32_t x = r + 1; x < nV; ++x)
+        if (PM_DEG(x) == 0) {
+          ready[nr++] = (pm_key){0, x};
+          break;
+        }
+    }
+    start -= nr;
+    qsort(ready, (size_t)nr, sizeof(pm_key), pm_by_deg_desc);
+    while (nr > 0) queue[tail++] = ready[--nr].id;
+    range[2 * scan] = (int32_t)start;
+    range[2 * scan + 1] = (int32_t)end;
+  }
+  if (scan != 0) rc = -4; /* the order must place every vertex */
+#undef PM_COUNT
+#undef PM_DEG
+out:
+  free(u_off);
+  free(u_adj);
+  free(pos);
+  free(cnt);
+  free(touched);
+  free(indeg);
+  free(queue);
+  free(ready);
+  return rc;
+}
+
+/* common neighbours N(L) of a non-empty ascending L (U ids), ascending V ids */
+static int64_t pm_common(const og* g, const uint32_t* L, int64_t nL, uint32_t* out) {
+  int64_t n = NU_DEG(g, L[0]);
+  memcpy(out, NU_ROW(g, L[0]), (size_t)n * 4);
+  for (int64_t j = 1; j < nL && n > 0; ++j) {
+    const uint32_t* row = NU_ROW(g, L[j]);
+    int64_t d = NU_DEG(g, L[j]), k = 0, e = 0;
+    for (int64_t i = 0; i < n; ++i) { /* in place: k <= i */
+      while (e < d && row[e] < out[i]) ++e;
+      if (e < d && row[e] == out[i]) out[k++] = out[i];
+    }
+    n = k;
+  }
+  return n;
+}
+
+/* PmbeAdvFinder::biclique_find (baselines/MBE.zip:MBE/PMBE.cpp:300-354).  C
+ * ascends and is consumed from its back.  The range a consumed candidate
+ * released joins this call's range queue; a candidate inside the range at the
+ * queue's head is skipped, since its neighbourhood lies inside that of a
+ * candidate already done.  A node is maximal when N(L') \ R lies within C. */
+static int pm_find(mb_ws* w, const uint32_t* L, int64_t nL, const uint32_t* R, int64_t nR,
+                   const uint32_t* C, int64_t nC) {
+  const og* g = w->g;
+  size_t base = w->arena_n;
+  int32_t* ex = (int32_t*)mb_alloc(w, 2 * (size_t)nC + 2);
+  if (!ex) return -1;
+  int64_t qh = 0, qt = 0;
+  while (nC > 0) {
+    if (w->deadline > 0 && (++w->tick & 255u) == 0 && wall() > w->deadline) w->cut = 1;
+    if (w->cut) break;
+    uint32_t x = C[nC - 1];
+    const int32_t* p = w->range + 2 * (size_t)x;
+    if (p[0] <= p[1]) {
+      ex[2 * qt] = p[0];
+      ex[2 * qt + 1] = p[1];
+      ++qt;
+    }
+    while (qh < qt && ex[2 * qh] > (int32_t)x) ++qh;
+    if (qh < qt && ex[2 * qh + 1] >= (int32_t)x) {
+      w->res.skipped++;
+      --nC;
+      continue;
+    }
+    size_t mark = w->arena_n;
+    uint32_t* Lp = mb_alloc(w, (size_t)nL + 1);
+    if (!Lp) return -1;
+    int64_t nLp = intersect(L, nL, NV_ROW(g, x), NV_DEG(g, x), Lp);
+    if (nLp == 0) {
+      w->arena_n = mark;
+      --nC;
+      continue;
+    }
+    w->res.nodes++;
+    uint32_t* Rp = mb_alloc(w, (size_t)NU_DEG(g, Lp[0]) + 1);
+    if (!Rp) return -1;
+    int64_t nRp = pm_common(g, Lp, nLp, Rp);
+    uint32_t* Ra = mb_alloc(w, (size_t)nRp + 1);
+    if (!Ra) return -1;
+    int64_t nRa = except(Rp, nRp, R, nR, Ra);
+    if (intersect_cnt(Ra, nRa, C, nC) != nRa) {
+      w->res.non_maximal++;
+    } else {
+      uint32_t* Cp = mb_alloc(w, (size_t)nC + 1);
+      if (!Cp) return -1;
+      int64_t nCp = 0;
+      for (int64_t i = 0; i < nC; ++i) {
+        uint32_t c = C[i];
+        if (c == x) continue;
+        int64_t nc = intersect_cnt(Lp, nLp, NV_ROW(g, c), NV_DEG(g, c));
+        if (nc > 0 && nc < nLp) Cp[nCp++] = c;
+      }
+      if (nCp > 0)
+        if (pm_find(w, Lp, nLp, Rp, nRp, Cp, nCp) < 0) return -1;
+      uint64_t sL = 0, sR = 0;
+      for (int64_t j = 0; j < nLp; ++j) sL += g->kU[Lp[j]];
+      for (int64_t j = 0; j < nRp; ++j) sR += g->kV[Rp[j]];
+      add_biclique(&w->res, hash_internal(g, sL, sR));
+      record(g, w->list, Lp, (size_t)nLp, Rp, (size_t)nRp);
+    }
+    w->arena_n = mark;
+    --nC;
+  }
+  w->arena_n = base;
+  return 0;
+}
+
+/* PmbeAdvFinder::Execute, one top-level vertex (baselines/MBE.zip:MBE/
+ * PMBE.cpp:258-298): L = N(v), R = N(L); canonical when no lower id is in R;
+ * then the search over the higher 2-hop neighbours not in R, ascending. */
+static int pm_root(mb_ws* w, uint32_t v) {
+  const og* g = w->g;
+  const uint32_t* L = NV_ROW(g, v);
+  int64_t nL = NV_DEG(g, v);
+  w->res.tasks++;
+  if (nL == 0) return 0;
+  w->res.nodes++;
+  int64_t nt = 0;
+  for (int64_t j = 0; j < nL; ++j) {
+    const uint32_t* row = NU_ROW(g, L[j]);
+    int64_t d = NU_DEG(g, L[j]);
+    for (int64_t e = 0; e < d; ++e)
+      if (w->cmap[row[e]]++ == 0) w->touched[nt++] = row[e];
+  }
+  int canonical = 1;
+  for (int64_t i = 0; i < nt; ++i)
+    if (w->cmap[w->touched[i]] == nL && w->touched[i] < v) canonical = 0;
+  if (!canonical) {
+    w->res.non_maximal++;
+    for (int64_t i = 0; i < nt; ++i) w->cmap[w->touched[i]] = 0;
+    return 0;
+  }
+  qsort(w->touched, (size_t)nt, 4, cmp_u32);
+  size_t mark = w->arena_n;
+  uint32_t* R = mb_alloc(w, (size_t)nt + 1);
+  uint32_t* C = mb_alloc(w, (size_t)nt + 1);
+  if (!R || !C) return -1;
+  int64_t nR = 0, nC = 0;
+  uint64_t sR = 0, sL = 0;
+  for (int64_t i = 0; i < nt; ++i) {
+    uint32_t y = w->touched[i];
+    if (w->cmap[y] == nL) {
+      R[nR++] = y;
+      sR += g->kV[y];
+    } else if (y > v) {
+      C[nC++] = y;
+    }
+    w->cmap[y] = 0;
+  }
+  for (int64_t j = 0; j < nL; ++j) sL += g->kU[L[j]];
+  add_biclique(&w->res, hash_internal(g, sL, sR));
+  record(g, w->list, L, (size_t)nL, R, (size_t)nR);
+  int rc = pm_find(w, L, nL, R, nR, C, nC);
+  w->arena_n = mark;
+  return rc;
+}
+
+/* ------------------------------------------------------------------------ */
 /* threading                                                                 */
 
 typedef struct {
-  int kind; /* 0 gmbe, 1 mbea, 2 imbea */
+  int kind; /* 0 gmbe, 1 mbea, 2 imbea, 3 pmbe */
   const og* g;
+  const int32_t* range; /* pmbe only */
   gm_params gp;
   int want_list;
   uint32_t stride, phase;
@@ -964,6 +1244,7 @@ static void* run_worker(void* arg) {
     w.arena_cap = (size_t)1 << 26; /* 256 MB of u32 per thread */
     w.arena = malloc(w.arena_cap * 4);
     w.deadline = j->deadline;
+    w.range = j->range;
     if (j->want_list) w.list = orc_list_new();
     if (!w.cmap || !w.touched || !w.arena) {
       wk->rc = -1;
@@ -974,7 +1255,8 @@ static void* run_worker(void* arg) {
         if (t >= j->n_roots || j->failed) break;
         if (j->deadline > 0 && wall() > j->deadline) break;
         uint32_t v = (uint32_t)(g->nV - 1 - ((uint64_t)t * j->stride + j->phase));
-        if ((j->kind == 2 ? im_root(&w, v) : mb_root(&w, v)) < 0) {
+        int r = j->kind == 3 ? pm_root(&w, v) : (j->kind == 2 ? im_root(&w, v) : mb_root(&w, v));
+        if (r < 0) {
           wk->rc = -2;
           j->failed = 1;
           break;
@@ -992,8 +1274,8 @@ static void* run_worker(void* arg) {
 
 static int run(int kind, uint32_t nU, uint32_t nV, const int64_t* v_off, const uint32_t* v_adj,
                const uint32_t* u_orig, const uint32_t* v_orig, int swapped, gm_params gp,
-               int nthreads, uint32_t stride, uint32_t phase, double time_limit, orc_result* out,
-               orc_list* list) {
+               int nthreads, uint32_t stride, uint32_t phase, double time_limit,
+               const int32_t* range, orc_result* out, orc_list* list) {
   /* roots: processing ranks phase, phase + stride, ... (phase may exceed the
    * stride, which starts the sample at a later rank) */
   if (!out || !v_off || nthreads < 1 || stride < 1) return -1;
@@ -1008,6 +1290,7 @@ static int run(int kind, uint32_t nU, uint32_t nV, const int64_t* v_off, const u
   memset(&j, 0, sizeof(j));
   j.kind = kind;
   j.g = &g;
+  j.range = range;
   j.gp = gp;
   j.want_list = list != NULL;
   j.stride = stride;
@@ -1038,7 +1321,8 @@ static int run(int kind, uint32_t nU, uint32_t nV, const int64_t* v_off, const u
     out->nodes += r->nodes;
     out->b_alg += r->b_alg;
     out->tasks += r->tasks;
-    out->roots += r->roots;
+    out->splits += r->splits;
+    out->skipped += r->skipped;
     if (wk[i].rc) err = wk[i].rc;
     if (wk[i].list) {
       if (list) list_append(list, wk[i].list);
@@ -1060,7 +1344,7 @@ int orc_gmbe(uint32_t nU, uint32_t nV, const int64_t* v_off, const uint32_t* v_a
              uint32_t phase, double time_limit, orc_result* out, orc_list* list) {
   gm_params p = {bound_height, bound_size, prune, max_prefix < 1 ? 4 : (max_prefix > 15 ? 15 : max_prefix)};
   return run(0, nU, nV, v_off, v_adj, u_orig, v_orig, swapped, p, nthreads, stride, phase,
-             time_limit, out, list);
+             time_limit, NULL, out, list);
 }
 
 int orc_imbea(uint32_t nU, uint32_t nV, const int64_t* v_off, const uint32_t* v_adj,
@@ -1069,7 +1353,53 @@ int orc_imbea(uint32_t nU, uint32_t nV, const int64_t* v_off, const uint32_t* v_
               orc_list* list) {
   gm_params p = {-1, 0, 1, 4};
   return run(2, nU, nV, v_off, v_adj, u_orig, v_orig, swapped, p, nthreads, stride, phase,
-             time_limit, out, list);
+             time_limit, NULL, out, list);
+}
+
+/* PMBE: V is renumbered by the containment order first; out->seconds times
+ * the search only.  Processing rank t is the t-th vertex Kahn's algorithm
+ * placed.  Hashes and lists carry original ids. */
+int orc_pmbe(uint32_t nU, uint32_t nV, const int64_t* v_off, const uint32_t* v_adj,
+             const uint32_t* u_orig, const uint32_t* v_orig, int swapped, int nthreads,
+             uint32_t stride, uint32_t phase, double time_limit, orc_result* out,
+             orc_list* list) {
+  if (!out || !v_off) return -1;
+  int64_t E = v_off[nV];
+  uint32_t* rename = malloc(((size_t)nV + 1) * 4);
+  int32_t* range = malloc(((size_t)nV + 1) * 8);
+  int64_t* n_off = malloc(((size_t)nV + 1) * sizeof(int64_t));
+  uint32_t* n_adj = malloc((size_t)(E ? E : 1) * 4);
+  uint32_t* n_orig = malloc(((size_t)nV + 1) * 4);
+  int rc = -1;
+  if (rename && range && n_off && n_adj && n_orig) {
+    for (uint32_t v = 0; v < nV; ++v)
+      for (int64_t e = v_off[v]; e < v_off[v + 1]; ++e)
+        if (v_adj[e] >= nU || (e > v_off[v] && v_adj[e] <= v_adj[e - 1])) {
+          rc = -2;
+          goto out;
+        }
+    rc = pm_order(nU, nV, v_off, v_adj, rename, range);
+    if (rc == 0) {
+      n_off[0] = 0;
+      for (uint32_t i = 0; i < nV; ++i) {
+        uint32_t o = rename[i];
+        int64_t d = v_off[o + 1] - v_off[o];
+        memcpy(n_adj + n_off[i], v_adj + v_off[o], (size_t)d * 4);
+        n_off[i + 1] = n_off[i] + d;
+        n_orig[i] = v_orig ? v_orig[o] : o;
+      }
+      gm_params p = {-1, 0, 1, 4};
+      rc = run(3, nU, nV, n_off, n_adj, u_orig, n_orig, swapped, p, nthreads, stride, phase,
+               time_limit, range, out, list);
+    }
+  }
+out:
+  free(rename);
+  free(range);
+  free(n_off);
+  free(n_adj);
+  free(n_orig);
+  return rc;
 }
 
 int orc_mbea(uint32_t nU, uint32_t nV, const int64_t* v_off, const uint32_t* v_adj,
@@ -1078,7 +1408,7 @@ int orc_mbea(uint32_t nU, uint32_t nV, const int64_t* v_off, const uint32_t* v_a
              orc_list* list) {
   gm_params p = {-1, 0, 1, 4};
   return run(1, nU, nV, v_off, v_adj, u_orig, v_orig, swapped, p, nthreads, stride, phase,
-             time_limit, out, list);
+             time_limit, NULL, out, list);
 }
 
 /* Definition check: every subset R of V. */

@@ -93,5 +93,24 @@ def kat_graph(name, **opt):
     return BiGraph.from_rows(nU, rows, **opt), cnt
 
 
+def nested_rows(nU, nV, seed):
+    """Rows of which most are subsets of an earlier row (sometimes with one
+    extra vertex), shuffled: long chains of neighbourhood containment, which
+    PMBE's containment order and its skip rule act on."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for _ in range(nV):
+        if rows and rng.random() < 0.7:
+            par = rows[int(rng.integers(len(rows)))]
+            keep = rng.random(len(par)) < rng.uniform(0.3, 0.95)
+            r = {u for u, k in zip(par, keep) if k}
+            if rng.random() < 0.2:
+                r.add(int(rng.integers(nU)))
+        else:
+            r = {int(u) for u in rng.choice(nU, int(rng.integers(1, nU)), replace=False)}
+        rows.append(sorted(r))
+    return [rows[i] for i in rng.permutation(nV)]
+
+
 def random_graph(nU, nV, p, seed, **opt):
     return BiGraph.from_rows(nU, random_rows(nU, nV, p, seed), **opt)

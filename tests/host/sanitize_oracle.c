@@ -1,6 +1,6 @@
 /* Host-only driver of the CPU oracle (oracle/oracle.c), built with
  * AddressSanitizer and UndefinedBehaviorSanitizer by
- * tests/test_host_sanitizers.py: the four enumerators on seeded random small
+ * tests/test_host_sanitizers.py: the five enumerators on seeded random small
  * graphs must agree, with list output on, through their worker threads. */
 #include <stdint.h>
 #include <stdio.h>
@@ -8,7 +8,7 @@
 #include <string.h>
 
 typedef struct {
-  uint64_t maximal, non_maximal, hash_sum, hash_xor, nodes, b_alg, tasks, roots;
+  uint64_t maximal, non_maximal, hash_sum, hash_xor, nodes, b_alg, tasks, splits, skipped;
   double seconds;
 } orc_result;
 typedef struct orc_list orc_list;
@@ -20,6 +20,8 @@ int orc_mbea(uint32_t, uint32_t, const int64_t*, const uint32_t*, const uint32_t
              int, int, uint32_t, uint32_t, double, orc_result*, orc_list*);
 int orc_imbea(uint32_t, uint32_t, const int64_t*, const uint32_t*, const uint32_t*,
               const uint32_t*, int, int, uint32_t, uint32_t, double, orc_result*, orc_list*);
+int orc_pmbe(uint32_t, uint32_t, const int64_t*, const uint32_t*, const uint32_t*,
+             const uint32_t*, int, int, uint32_t, uint32_t, double, orc_result*, orc_list*);
 int orc_brute(uint32_t, uint32_t, const int64_t*, const uint32_t*, const uint32_t*, const uint32_t*,
               int, orc_result*, orc_list*);
 
@@ -49,25 +51,29 @@ int main(void) {
         if (next_u32() % 100 < pct) adj[k++] = u;
       off[v + 1] = k;
     }
-    orc_result a, b, c, d, e;
+    orc_result a, b, c, d, e, f;
     orc_list* la = orc_list_new();
     orc_list* lc = orc_list_new();
+    orc_list* lf = orc_list_new();
     int rc = orc_gmbe(nU, nV, off, adj, uo, vo, 0, -1, 0, 1, 4, 2, 1, 0, 0.0, &a, la);
     rc |= orc_gmbe(nU, nV, off, adj, uo, vo, 0, 1, 2, 1, 4, 2, 1, 0, 0.0, &b, NULL);
     rc |= orc_mbea(nU, nV, off, adj, uo, vo, 0, 2, 1, 0, 0.0, &c, lc);
     rc |= orc_imbea(nU, nV, off, adj, uo, vo, 0, 2, 1, 0, 0.0, &d, NULL);
     rc |= orc_brute(nU, nV, off, adj, uo, vo, 0, &e, NULL);
+    rc |= orc_pmbe(nU, nV, off, adj, uo, vo, 0, 2, 1, 0, 0.0, &f, lf);
     if (rc || a.maximal != e.maximal || b.maximal != e.maximal || c.maximal != e.maximal ||
         d.maximal != e.maximal || a.hash_sum != e.hash_sum || d.hash_sum != e.hash_sum ||
-        c.hash_xor != e.hash_xor) {
-      printf("mismatch on graph %d (%u x %u): rc %d, %llu %llu %llu %llu %llu\n", t, nU, nV, rc,
-             (unsigned long long)a.maximal, (unsigned long long)b.maximal,
+        c.hash_xor != e.hash_xor || f.maximal != e.maximal || f.hash_sum != e.hash_sum ||
+        f.hash_xor != e.hash_xor) {
+      printf("mismatch on graph %d (%u x %u): rc %d, %llu %llu %llu %llu %llu %llu\n", t, nU, nV,
+             rc, (unsigned long long)a.maximal, (unsigned long long)b.maximal,
              (unsigned long long)c.maximal, (unsigned long long)d.maximal,
-             (unsigned long long)e.maximal);
+             (unsigned long long)e.maximal, (unsigned long long)f.maximal);
       ++fails;
     }
     orc_list_free(la);
     orc_list_free(lc);
+    orc_list_free(lf);
     free(off);
     free(adj);
     free(uo);

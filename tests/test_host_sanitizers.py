@@ -32,7 +32,7 @@ def test_loader_and_generators_are_sanitizer_clean(tmp_path):
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="no host C compiler")
 def test_oracle_is_sanitizer_clean(tmp_path):
-    """The CPU oracle itself: the four enumerators agree on random small
+    """The CPU oracle itself: the five enumerators agree on random small
     graphs, lists on, worker threads on, under ASan and UBSan."""
     exe = tmp_path / "drive_oracle"
     cmd = ["gcc", "-std=c11", "-g", "-O1", "-fsanitize=address,undefined",

@@ -5,8 +5,10 @@
 * closed forms (complete, matching, crown = 2^n - 2 with its full list, chain,
   stars, disjoint unions);
 * the definition itself: brute force over every subset of V on small graphs;
-* three-way agreement of the GMBE-order restatement, the MBEA restatement and
-  brute force, with full lists;
+* agreement of the GMBE-order restatement, the MBEA, iMBEA and PMBE
+  restatements and brute force, with full lists;
+* PMBE's containment skip on graphs built of nested neighbourhoods, with two
+  planted mistakes in the order's ranges shown to fail this check;
 * invariance of the digest under side swap, vertex order, task splitting and
   PRUNE_EN (all change the traversal, none may change the result).
 """
@@ -84,7 +86,8 @@ REF = json.load(open(os.path.join(HERE, "golden", "ref_counts.json")))["graphs"]
 
 @pytest.mark.parametrize("name", sorted(REF))
 def test_oracle_matches_reference_cpu_counts(name):
-    """Counts of the reference's own MbeaAdvFinder / ImbeaAdvFinder / PmbeAdvFinder."""
+    """Counts of the reference's own MbeaAdvFinder / ImbeaAdvFinder / PmbeAdvFinder
+    (its -s 0 / 1 / 2, frozen per graph) against the four restatements."""
     ent = REF[name]
     g = build_recipe(ent["recipe"]).derive(uvtrans=1, order=1, fast_mode=1)
     og = g.oracle_graph()
@@ -92,13 +95,15 @@ def test_oracle_matches_reference_cpu_counts(name):
     if ent["count"] < 5_000_000:
         assert oracle.mbea(og, threads=8).maximal == ent["count"]
         assert oracle.imbea(og, threads=8).maximal == ent["count"]
+    if ent["count"] < 1_000_000:
+        assert oracle.pmbe(og, threads=8).maximal == ent["ref_finders"]["s2"]
 
 
 @pytest.mark.parametrize("name", sorted(graphs.KATS))
 def test_closed_forms(name):
     g, cnt = graphs.kat_graph(name)
     og = g.oracle_graph()
-    for run in (oracle.gmbe, oracle.mbea):
+    for run in (oracle.gmbe, oracle.mbea, oracle.imbea, oracle.pmbe):
         assert run(og).maximal == cnt
     if g.stats()["V_size"] <= 24 and g.stats()["U_size"] <= 64:
         assert oracle.brute(og).maximal == cnt
@@ -128,11 +133,34 @@ def test_three_way_agreement_small(seed):
     rg, lg = oracle.gmbe(og, want_list=True)
     rm, lm = oracle.mbea(og, want_list=True)
     ri, li = oracle.imbea(og, want_list=True)
+    rp, lp = oracle.pmbe(og, want_list=True)
     sb = records_to_set(lb)
     assert records_to_set(lg) == sb
     assert records_to_set(lm) == sb
     assert records_to_set(li) == sb
-    assert rb.digest() == rg.digest() == rm.digest() == ri.digest() == digest_of(sb)
+    assert records_to_set(lp) == sb
+    assert rb.digest() == rg.digest() == rm.digest() == ri.digest() == rp.digest() == \
+        digest_of(sb)
+
+
+def test_pmbe_containment_skip_on_nested_graphs():
+    """PMBE skips a candidate inside the range of vertices a done candidate
+    released; the full list must still equal brute force.  Nested rows make
+    the skip frequent; a range one id too wide, or stored at the neighbouring
+    id, changes the list on about half of these graphs."""
+    from gmbe_amd import records_to_set
+    skipped = with_skips = 0
+    for seed in range(60):
+        rng = np.random.default_rng(seed)
+        nU, nV = int(rng.integers(4, 30)), int(rng.integers(4, 20))
+        og = BiGraph.from_rows(nU, graphs.nested_rows(nU, nV, seed)).oracle_graph()
+        rb, lb = oracle.brute(og, want_list=True)
+        rp, lp = oracle.pmbe(og, want_list=True)
+        assert records_to_set(lp) == records_to_set(lb), seed
+        assert rp.digest() == rb.digest()
+        skipped += rp.skipped
+        with_skips += rp.skipped > 0
+    assert with_skips >= 20 and skipped >= 200
 
 
 @pytest.mark.parametrize("opt", [(0, 0), (1, 1), (2, 2), (1, 0), (0, 3)])
