@@ -1,5 +1,5 @@
 """Host logic that needs no GPU: the seeded workload generators, the ego cut of
-the served workload, and the benchmark's command-line contract."""
+the served workload, the benchmark's command-line contract and its CPU legs."""
 import os
 import subprocess
 import sys
@@ -67,6 +67,27 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_bench_cpu_baseline_legs():
+    """The CPU-baseline legs of the benchmark line (MbeaFinder, ImbeaAdvFinder
+    and PmbeAdvFinder restatements) on a small graph: every field the line
+    carries, and the whole sample's count when the time is ample."""
+    sys.path.insert(0, ROOT)
+    import bench
+    import oracle
+    g = workloads.run_graph(1)
+    total = oracle.mbea(g.oracle_graph(), threads=4).maximal
+    for algo in ("mbea", "imbea", "pmbe"):
+        b = bench.cpu_baseline(g, 0, 30.0, algo=algo)
+        assert set(b) == {"value", "unit", "cores", "kind", "sample"}
+        assert b["unit"] == "bicliques/s" and b["kind"] == "port" and b["cores"] >= 1
+        assert b["value"] > 0
+        # stride 97 from rank 0: MBEA and iMBEA walk all 97 phases, PMBE one
+        if algo != "pmbe":
+            assert f"{total} maximal bicliques" in b["sample"], b["sample"]
+        else:
+            assert "containment order" in b["sample"]
 
 
 def test_finder_knob_errors_are_codes():
