@@ -112,5 +112,31 @@ def nested_rows(nU, nV, seed):
     return [rows[i] for i in rng.permutation(nV)]
 
 
+def pmbe_recipe_rows(rec):
+    """(nU, rows) of a small seeded graph, at most 16 vertices a side."""
+    rng = np.random.default_rng(rec["seed"])
+    nU, nV = int(rng.integers(3, 17)), int(rng.integers(3, 17))
+    if rec["kind"] == "random":
+        return nU, random_rows(nU, nV, float(rng.uniform(0.1, 0.7)), rec["seed"])
+    return nU, nested_rows(nU, nV, rec["seed"])
+
+
+def mbe_orient(nU, rows):
+    """The graph as the loader of baselines/MBE.zip sees it (MBE/BiGraph.cpp:
+    38-79): each non-empty row is an L vertex (empty rows dropped), the items
+    0..max are R vertices, and the sides swap when the largest L degree
+    exceeds the largest R degree.  Returned with R as the V side, the side
+    its finders take their roots from."""
+    rows_nz = [r for r in rows if r]
+    nR = max((max(r) for r in rows_nz), default=-1) + 1
+    cols = [[] for _ in range(nR)]
+    for i, r in enumerate(rows_nz):
+        for u in r:
+            cols[u].append(i)
+    if max(map(len, rows_nz), default=0) > max(map(len, cols), default=0):
+        return BiGraph.from_rows(nR, rows_nz)
+    return BiGraph.from_rows(len(rows_nz), cols)
+
+
 def random_graph(nU, nV, p, seed, **opt):
     return BiGraph.from_rows(nU, random_rows(nU, nV, p, seed), **opt)

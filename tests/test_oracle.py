@@ -8,7 +8,9 @@
 * agreement of the GMBE-order restatement, the MBEA, iMBEA and PMBE
   restatements and brute force, with full lists;
 * PMBE's containment skip on graphs built of nested neighbourhoods, with two
-  planted mistakes in the order's ranges shown to fail this check;
+  planted mistakes in the order's ranges shown to fail this check, and its
+  processing-node counts equal to the reference binary's on small graphs
+  (tests/golden/pmbe_nodes.json);
 * invariance of the digest under side swap, vertex order, task splitting and
   PRUNE_EN (all change the traversal, none may change the result).
 """
@@ -141,6 +143,25 @@ def test_three_way_agreement_small(seed):
     assert records_to_set(lp) == sb
     assert rb.digest() == rg.digest() == rm.digest() == ri.digest() == rp.digest() == \
         digest_of(sb)
+
+
+PMBE_NODES = json.load(open(os.path.join(HERE, "golden", "pmbe_nodes.json")))["graphs"]
+
+
+def test_pmbe_walks_the_reference_traversal():
+    """Maximal AND processing-node counts of the reference's own PmbeAdvFinder
+    (frozen per graph) on 120 small random and nested graphs, each oriented
+    as its loader orients it.  With at most 16 root-side vertices the
+    reference's containment order is fixed, so equal node counts pin the
+    order, the released ranges, the skip rule and the maximality test, not
+    only the result."""
+    skipped = 0
+    for name, ent in sorted(PMBE_NODES.items()):
+        nU, rows = graphs.pmbe_recipe_rows(ent["recipe"])
+        r = oracle.pmbe(graphs.mbe_orient(nU, rows).oracle_graph())
+        assert (r.maximal, r.nodes) == (ent["maximal"], ent["nodes"]), name
+        skipped += r.skipped
+    assert skipped > 0
 
 
 def test_pmbe_containment_skip_on_nested_graphs():
