@@ -733,14 +733,28 @@ static int mb_root(mb_ws* w, uint32_t v) {
 
 typedef struct {
   uint32_t id;
+  uint32_t pos; /* index in the parent's P */
   int64_t cnt;
 } im_pair;
 
+/* top level: overlap descending, then id descending (the comparator of
+ * baselines/MBE.zip:MBE/BicliqueFinder.cpp:268-272, a total order) */
 static int im_by_count_desc(const void* a, const void* b) {
   const im_pair* x = (const im_pair*)a;
   const im_pair* y = (const im_pair*)b;
   if (x->cnt != y->cnt) return x->cnt > y->cnt ? -1 : 1;
   return x->id > y->id ? -1 : (x->id < y->id ? 1 : 0);
+}
+
+/* inside the search: overlap descending only (BicliqueFinder.cpp:329-332);
+ * std::sort leaves equal overlaps in an unspecified order, and keeps P's
+ * order for up to 16 elements (libstdc++ sorts those by insertion): ties
+ * keep P's order here */
+static int im_by_count_desc_stable(const void* a, const void* b) {
+  const im_pair* x = (const im_pair*)a;
+  const im_pair* y = (const im_pair*)b;
+  if (x->cnt != y->cnt) return x->cnt > y->cnt ? -1 : 1;
+  return x->pos < y->pos ? -1 : (x->pos > y->pos ? 1 : 0);
 }
 
 /* n pairs from the u32 arena, 8-byte aligned */
@@ -773,6 +787,7 @@ static int64_t except(const uint32_t* a, int64_t na, const uint32_t* b, int64_t 
 static int im_find(mb_ws* w, const uint32_t* L, int64_t nL, const uint32_t* R, int64_t nR,
                    uint64_t sR, uint32_t* P, int64_t nP, uint32_t* Q, int64_t nQ) {
   const og* g = w->g;
+  const int64_t nQ0 = nQ; /* Q[nQ0..) holds this call's own additions */
   while (nP > 0) {
     if (w->deadline > 0 && (++w->tick & 255u) == 0 && wall() > w->deadline) w->cut = 1;
     if (w->cut) return 0;
@@ -821,12 +836,13 @@ static int im_find(mb_ws* w, const uint32_t* L, int64_t nL, const uint32_t* R, i
         if (intersect_cnt(Lc, nLc, NV_ROW(g, p), NV_DEG(g, p)) == 0) Cx[nCx++] = p;
       } else if (nc > 0) {
         pp[npp].id = p;
+        pp[npp].pos = (uint32_t)i;
         pp[npp].cnt = nc;
         ++npp;
       }
     }
     if (npp > 0) {
-      qsort(pp, (size_t)npp, sizeof(im_pair), im_by_count_desc);
+      qsort(pp, (size_t)npp, sizeof(im_pair), im_by_count_desc_stable);
       uint32_t* Pp = mb_alloc(w, (size_t)npp + 1);
       if (!Pp) return -1;
       for (int64_t i = 0; i < npp; ++i) Pp[i] = pp[i].id;
@@ -839,15 +855,31 @@ static int im_find(mb_ws* w, const uint32_t* L, int64_t nL, const uint32_t* R, i
       qsort(Rp, (size_t)nRp, 4, cmp_u32);
       record(g, w->list, Lp, (size_t)nLp, Rp, (size_t)nRp);
     }
-    /* P = P \ C (order kept), Q += C */
-    int64_t k = 0;
-    for (int64_t i = 0; i < nP; ++i) {
-      int in_c = 0;
-      for (int64_t c = 1; c < nCx && !in_c; ++c) in_c = Cx[c] == P[i];
-      if (!in_c) P[k++] = P[i];
+    /* P = seq_except(P, C) as the reference computes it
+     * (BicliqueFinder.cpp:345, Utility.cpp:97-113): a merge walk that assumes
+     * both lists ascend, applied to P in overlap order and C in P's order.
+     * It drops only members of C and may keep some; a kept one is rejected
+     * as non-maximal when it comes up (x, its twin on L, is in Q), so only
+     * the node count follows the walk, and the result does not. */
+    int64_t k = 0, c = 0;
+    for (int64_t i = 0; i < nP;) {
+      if (c == nCx || P[i] < Cx[c]) {
+        P[k++] = P[i++];
+      } else if (P[i] == Cx[c]) {
+        ++i;
+        ++c;
+      } else {
+        ++c;
+      }
     }
     nP = k;
-    for (int64_t c = 0; c < nCx; ++c) Q[nQ++] = Cx[c];
+    /* Q += C.  A member kept in P can be in a later C again; Q holds it once
+     * (a repeat changes no maximality test), so Q never outgrows P. */
+    for (int64_t j = 0; j < nCx; ++j) {
+      int seen = 0;
+      for (int64_t q = nQ0; q < nQ && !seen; ++q) seen = Q[q] == Cx[j];
+      if (!seen) Q[nQ++] = Cx[j];
+    }
     w->arena_n = mark;
   }
   return 0;
@@ -862,8 +894,8 @@ static int im_root(mb_ws* w, uint32_t v) {
   const uint32_t* L = NV_ROW(g, v);
   int64_t nL = NV_DEG(g, v);
   w->res.tasks++;
-  w->res.nodes++;
   if (nL == 0) return 0;
+  w->res.nodes++;
   int64_t nt = 0;
   for (int64_t j = 0; j < nL; ++j) {
     const uint32_t* row = NU_ROW(g, L[j]);
@@ -898,6 +930,7 @@ static int im_root(mb_ws* w, uint32_t v) {
       Q[nQ++] = y;
     } else {
       pp[npp].id = y;
+      pp[npp].pos = 0;
       pp[npp].cnt = c;
       ++npp;
     }

@@ -121,12 +121,12 @@ def pmbe_recipe_rows(rec):
     return nU, nested_rows(nU, nV, rec["seed"])
 
 
-def mbe_orient(nU, rows):
+def mbe_orient_rows(nU, rows):
     """The graph as the loader of baselines/MBE.zip sees it (MBE/BiGraph.cpp:
     38-79): each non-empty row is an L vertex (empty rows dropped), the items
     0..max are R vertices, and the sides swap when the largest L degree
-    exceeds the largest R degree.  Returned with R as the V side, the side
-    its finders take their roots from."""
+    exceeds the largest R degree.  Returned as (|L|, R rows): R is the side
+    its finders take their roots from, the V side here."""
     rows_nz = [r for r in rows if r]
     nR = max((max(r) for r in rows_nz), default=-1) + 1
     cols = [[] for _ in range(nR)]
@@ -134,8 +134,24 @@ def mbe_orient(nU, rows):
         for u in r:
             cols[u].append(i)
     if max(map(len, rows_nz), default=0) > max(map(len, cols), default=0):
-        return BiGraph.from_rows(nR, rows_nz)
-    return BiGraph.from_rows(len(rows_nz), cols)
+        return nR, rows_nz
+    return len(rows_nz), cols
+
+
+def mbe_orient(nU, rows):
+    """mbe_orient_rows as a graph (V = the loader's R side)."""
+    return BiGraph.from_rows(*mbe_orient_rows(nU, rows))
+
+
+def mbe_reorder2(nU, rows):
+    """mbe_orient, then R renumbered as BiGraph::Reorder(2) of
+    baselines/MBE.zip numbers it before ImbeaAdvFinder runs
+    (MBE/BiGraph.cpp:103-140): degree ascending, then the neighbour list
+    lexicographically; equal lists keep their order (std::sort sorts up to
+    16 elements by insertion in libstdc++)."""
+    nL, vrows = mbe_orient_rows(nU, rows)
+    order = sorted(range(len(vrows)), key=lambda i: (len(vrows[i]), vrows[i], i))
+    return BiGraph.from_rows(nL, [vrows[i] for i in order])
 
 
 def random_graph(nU, nV, p, seed, **opt):

@@ -9,8 +9,8 @@
   restatements and brute force, with full lists;
 * PMBE's containment skip on graphs built of nested neighbourhoods, with two
   planted mistakes in the order's ranges shown to fail this check, and its
-  processing-node counts equal to the reference binary's on small graphs
-  (tests/golden/pmbe_nodes.json);
+  processing-node counts of the PMBE and iMBEA restatements equal to the
+  reference binary's on small graphs (tests/golden/mbe_zip_nodes.json);
 * invariance of the digest under side swap, vertex order, task splitting and
   PRUNE_EN (all change the traversal, none may change the result).
 """
@@ -145,23 +145,33 @@ def test_three_way_agreement_small(seed):
         digest_of(sb)
 
 
-PMBE_NODES = json.load(open(os.path.join(HERE, "golden", "pmbe_nodes.json")))["graphs"]
+MBE_ZIP_NODES = json.load(open(os.path.join(HERE, "golden", "mbe_zip_nodes.json")))["graphs"]
 
 
 def test_pmbe_walks_the_reference_traversal():
     """Maximal AND processing-node counts of the reference's own PmbeAdvFinder
-    (frozen per graph) on 120 small random and nested graphs, each oriented
-    as its loader orients it.  With at most 16 root-side vertices the
-    reference's containment order is fixed, so equal node counts pin the
+    (-s 2, frozen per graph) on 120 small random and nested graphs, each
+    oriented as its loader orients it.  With at most 16 root-side vertices
+    the reference's containment order is fixed, so equal node counts pin the
     order, the released ranges, the skip rule and the maximality test, not
     only the result."""
     skipped = 0
-    for name, ent in sorted(PMBE_NODES.items()):
+    for name, ent in sorted(MBE_ZIP_NODES.items()):
         nU, rows = graphs.pmbe_recipe_rows(ent["recipe"])
         r = oracle.pmbe(graphs.mbe_orient(nU, rows).oracle_graph())
-        assert (r.maximal, r.nodes) == (ent["maximal"], ent["nodes"]), name
+        assert (r.maximal, r.nodes) == (ent["s2"]["maximal"], ent["s2"]["nodes"]), name
         skipped += r.skipped
     assert skipped > 0
+
+
+def test_imbea_walks_the_reference_traversal():
+    """The same for ImbeaAdvFinder (-s 1): the graph numbered as its
+    Reorder(2) numbers it, candidates in overlap order with ties kept, and
+    P minus the twins computed by the reference's merge walk."""
+    for name, ent in sorted(MBE_ZIP_NODES.items()):
+        nU, rows = graphs.pmbe_recipe_rows(ent["recipe"])
+        r = oracle.imbea(graphs.mbe_reorder2(nU, rows).oracle_graph())
+        assert (r.maximal, r.nodes) == (ent["s1"]["maximal"], ent["s1"]["nodes"]), name
 
 
 def test_pmbe_containment_skip_on_nested_graphs():
