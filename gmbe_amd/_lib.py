@@ -17,6 +17,7 @@ ERRORS = {
     -1: "GMBE_ERR_ARG", -2: "GMBE_ERR_IO", -3: "GMBE_ERR_FORMAT",
     -4: "GMBE_ERR_CAPACITY", -5: "GMBE_ERR_HIP", -6: "GMBE_ERR_NOMEM",
     -7: "GMBE_ERR_REJECTED", -8: "GMBE_ERR_UNSUPPORTED", -9: "GMBE_ERR_STATE", -10: "GMBE_ERR_DEADLINE",
+    -11: "GMBE_ERR_INTERNAL",
 }
 
 

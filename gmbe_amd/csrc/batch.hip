@@ -261,8 +261,8 @@ int autotune(gmbe_batch* b) {
 
 extern "C" {
 
-int gmbe_batch_create(const gmbe_graph* const* graphs, int n, int bound_height, int bound_size,
-                      gmbe_batch** out) {
+static int gmbe_batch_create_impl(const gmbe_graph* const* graphs, int n, int bound_height,
+                                  int bound_size, gmbe_batch** out) {
   if (!out || (n > 0 && !graphs)) return set_error(GMBE_ERR_ARG, "null argument");
   *out = nullptr;
   if (n < 1 || (uint32_t)n > mbe::kMaxGraphs)
@@ -300,7 +300,14 @@ int gmbe_batch_create(const gmbe_graph* const* graphs, int n, int bound_height, 
   return GMBE_OK;
 }
 
-int gmbe_batch_set(gmbe_batch* b, const char* key, int64_t value) {
+int gmbe_batch_create(const gmbe_graph* const* graphs, int n, int bound_height, int bound_size,
+                      gmbe_batch** out) {
+  return gmbe::guarded("gmbe_batch_create", [&] {
+    return gmbe_batch_create_impl(graphs, n, bound_height, bound_size, out);
+  });
+}
+
+static int gmbe_batch_set_impl(gmbe_batch* b, const char* key, int64_t value) {
   if (!b || !key) return set_error(GMBE_ERR_ARG, "null argument");
   const std::string k(key);
   auto range = [&](int64_t lo, int64_t hi) -> int {
@@ -368,7 +375,11 @@ int gmbe_batch_set(gmbe_batch* b, const char* key, int64_t value) {
   return GMBE_OK;
 }
 
-int gmbe_batch_get(const gmbe_batch* b, const char* key, int64_t* value) {
+int gmbe_batch_set(gmbe_batch* b, const char* key, int64_t value) {
+  return gmbe::guarded("gmbe_batch_set", [&] { return gmbe_batch_set_impl(b, key, value); });
+}
+
+static int gmbe_batch_get_impl(const gmbe_batch* b, const char* key, int64_t* value) {
   if (!b || !key || !value) return set_error(GMBE_ERR_ARG, "null argument");
   const std::string k(key);
   if (k == "waves_per_block") *value = b->lane.knobs.waves;
@@ -391,7 +402,11 @@ int gmbe_batch_get(const gmbe_batch* b, const char* key, int64_t* value) {
   return GMBE_OK;
 }
 
-int gmbe_batch_execute(gmbe_batch* b, gmbe_result* per_graph, gmbe_result* total) {
+int gmbe_batch_get(const gmbe_batch* b, const char* key, int64_t* value) {
+  return gmbe::guarded("gmbe_batch_get", [&] { return gmbe_batch_get_impl(b, key, value); });
+}
+
+static int gmbe_batch_execute_impl(gmbe_batch* b, gmbe_result* per_graph, gmbe_result* total) {
   if (!b) return set_error(GMBE_ERR_ARG, "null argument");
   if (hipSetDevice(b->eng.device) != hipSuccess) return set_error(GMBE_ERR_HIP, "hipSetDevice");
   std::vector<gmbe_result> res;
@@ -429,6 +444,12 @@ int gmbe_batch_execute(gmbe_batch* b, gmbe_result* per_graph, gmbe_result* total
     return set_error(GMBE_ERR_CAPACITY, "%llu task(s) exceeded a capacity; the result is incomplete",
                      (unsigned long long)t.overflow);
   return GMBE_OK;
+}
+
+int gmbe_batch_execute(gmbe_batch* b, gmbe_result* per_graph, gmbe_result* total) {
+  return gmbe::guarded("gmbe_batch_execute", [&] {
+    return gmbe_batch_execute_impl(b, per_graph, total);
+  });
 }
 
 void gmbe_batch_destroy(gmbe_batch* b) {

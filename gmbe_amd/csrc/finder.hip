@@ -492,8 +492,8 @@ static int finder_alloc(gmbe_finder* f) {
 
 extern "C" {
 
-int gmbe_finder_create(const gmbe_graph* g, int sel, int bound_height, int bound_size, int ngpus,
-                       gmbe_finder** out) {
+static int gmbe_finder_create_impl(const gmbe_graph* g, int sel, int bound_height, int bound_size,
+                                   int ngpus, gmbe_finder** out) {
   if (!g || !out) return set_error(GMBE_ERR_ARG, "null argument");
   *out = nullptr;
   if (sel < 0 || sel > 3) return set_error(GMBE_ERR_ARG, "sel must be 0..3 (got %d)", sel);
@@ -534,7 +534,14 @@ int gmbe_finder_create(const gmbe_graph* g, int sel, int bound_height, int bound
   return GMBE_OK;
 }
 
-int gmbe_finder_set(gmbe_finder* f, const char* key, int64_t value) {
+int gmbe_finder_create(const gmbe_graph* g, int sel, int bound_height, int bound_size, int ngpus,
+                       gmbe_finder** out) {
+  return gmbe::guarded("gmbe_finder_create", [&] {
+    return gmbe_finder_create_impl(g, sel, bound_height, bound_size, ngpus, out);
+  });
+}
+
+static int gmbe_finder_set_impl(gmbe_finder* f, const char* key, int64_t value) {
   if (!f || !key) return set_error(GMBE_ERR_ARG, "null argument");
   std::string k(key);
   auto range = [&](int64_t lo, int64_t hi) -> int {
@@ -578,7 +585,11 @@ int gmbe_finder_set(gmbe_finder* f, const char* key, int64_t value) {
   return GMBE_OK;
 }
 
-int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res) {
+int gmbe_finder_set(gmbe_finder* f, const char* key, int64_t value) {
+  return gmbe::guarded("gmbe_finder_set", [&] { return gmbe_finder_set_impl(f, key, value); });
+}
+
+static int gmbe_finder_execute_impl(gmbe_finder* f, gmbe_result* res) {
   if (!f || !res) return set_error(GMBE_ERR_ARG, "null argument");
   if (f->root_phase >= f->root_stride)
     return set_error(GMBE_ERR_ARG, "root_phase must be < root_stride");
@@ -740,7 +751,11 @@ int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res) {
   return GMBE_OK;
 }
 
-int gmbe_finder_exit_times(const gmbe_finder* f, double* out, size_t cap, size_t* n) {
+int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res) {
+  return gmbe::guarded("gmbe_finder_execute", [&] { return gmbe_finder_execute_impl(f, res); });
+}
+
+static int gmbe_finder_exit_times_impl(const gmbe_finder* f, double* out, size_t cap, size_t* n) {
   if (!f || !n) return set_error(GMBE_ERR_ARG, "null argument");
   if (f->exit_s.empty()) return set_error(GMBE_ERR_STATE, "no run with exit_stamps set");
   *n = f->exit_s.size();
@@ -748,7 +763,13 @@ int gmbe_finder_exit_times(const gmbe_finder* f, double* out, size_t cap, size_t
   return GMBE_OK;
 }
 
-int gmbe_finder_list(gmbe_finder* f, uint32_t* buf, size_t cap, size_t* used) {
+int gmbe_finder_exit_times(const gmbe_finder* f, double* out, size_t cap, size_t* n) {
+  return gmbe::guarded("gmbe_finder_exit_times", [&] {
+    return gmbe_finder_exit_times_impl(f, out, cap, n);
+  });
+}
+
+static int gmbe_finder_list_impl(gmbe_finder* f, uint32_t* buf, size_t cap, size_t* used) {
   if (!f || !used) return set_error(GMBE_ERR_ARG, "null argument");
   if (!f->ran || !f->list) return set_error(GMBE_ERR_STATE, "no list-mode run (set list_cap)");
   if (f->list_ovf)
@@ -783,11 +804,21 @@ int gmbe_finder_list(gmbe_finder* f, uint32_t* buf, size_t cap, size_t* used) {
   return GMBE_OK;
 }
 
-int gmbe_finder_stats(gmbe_finder* f, uint64_t* out, size_t n) {
+int gmbe_finder_list(gmbe_finder* f, uint32_t* buf, size_t cap, size_t* used) {
+  return gmbe::guarded("gmbe_finder_list", [&] {
+    return gmbe_finder_list_impl(f, buf, cap, used);
+  });
+}
+
+static int gmbe_finder_stats_impl(gmbe_finder* f, uint64_t* out, size_t n) {
   if (!f) return set_error(GMBE_ERR_ARG, "null finder");
   const int first = mbe::A_CYC_WAIT, avail = mbe::A_COUNT - mbe::A_CYC_WAIT;
   for (size_t i = 0; out && i < n && i < (size_t)avail; ++i) out[i] = f->prof[first + i];
   return avail;
+}
+
+int gmbe_finder_stats(gmbe_finder* f, uint64_t* out, size_t n) {
+  return gmbe::guarded("gmbe_finder_stats", [&] { return gmbe_finder_stats_impl(f, out, n); });
 }
 
 void gmbe_finder_destroy(gmbe_finder* f) {
@@ -810,7 +841,7 @@ void gmbe_finder_destroy(gmbe_finder* f) {
   delete f;
 }
 
-int gmbe_device_info(char* buf, size_t cap, int* cu_count) {
+static int gmbe_device_info_impl(char* buf, size_t cap, int* cu_count) {
   int n = 0;
   hipError_t e = hipGetDeviceCount(&n);
   if (e != hipSuccess || n == 0)
@@ -820,6 +851,12 @@ int gmbe_device_info(char* buf, size_t cap, int* cu_count) {
   if (buf && cap) snprintf(buf, cap, "%s (%s)", prop.name, prop.gcnArchName);
   if (cu_count) *cu_count = prop.multiProcessorCount;
   return GMBE_OK;
+}
+
+int gmbe_device_info(char* buf, size_t cap, int* cu_count) {
+  return gmbe::guarded("gmbe_device_info", [&] {
+    return gmbe_device_info_impl(buf, cap, cu_count);
+  });
 }
 
 }  // extern "C"

@@ -281,14 +281,18 @@ using namespace gmbe;
 
 extern "C" {
 
-int gmbe_generate(const gmbe_gen_params* p, gmbe_graph** out) {
+static int gmbe_generate_impl(const gmbe_gen_params* p, gmbe_graph** out) {
   if (!p || !out) return set_error(GMBE_ERR_ARG, "null argument");
   *out = nullptr;
   return generate(*p, out);
 }
 
-int gmbe_generate_small(uint64_t seed, uint32_t lo, uint32_t hi, double dlo, double dhi,
-                        double gamma, gmbe_graph** out) {
+int gmbe_generate(const gmbe_gen_params* p, gmbe_graph** out) {
+  return gmbe::guarded("gmbe_generate", [&] { return gmbe_generate_impl(p, out); });
+}
+
+static int gmbe_generate_small_impl(uint64_t seed, uint32_t lo, uint32_t hi, double dlo, double dhi,
+                                    double gamma, gmbe_graph** out) {
   if (!out || lo == 0 || hi < lo || dlo <= 0 || dhi < dlo)
     return set_error(GMBE_ERR_ARG, "bad small-graph parameters");
   Rng r(seed);
@@ -312,7 +316,15 @@ int gmbe_generate_small(uint64_t seed, uint32_t lo, uint32_t hi, double dlo, dou
   return gmbe_generate(&p, out);
 }
 
-int gmbe_ego_subgraph(const gmbe_graph* g, uint32_t u, uint64_t max_edges, gmbe_graph** out) {
+int gmbe_generate_small(uint64_t seed, uint32_t lo, uint32_t hi, double dlo, double dhi,
+                        double gamma, gmbe_graph** out) {
+  return gmbe::guarded("gmbe_generate_small", [&] {
+    return gmbe_generate_small_impl(seed, lo, hi, dlo, dhi, gamma, out);
+  });
+}
+
+static int gmbe_ego_subgraph_impl(const gmbe_graph* g, uint32_t u, uint64_t max_edges,
+                                  gmbe_graph** out) {
   // g is in file orientation: rows = V (original V ids), items = U.
   if (!g || !out) return set_error(GMBE_ERR_ARG, "null argument");
   if (g->swapped) return set_error(GMBE_ERR_ARG, "ego: graph must be in file orientation");
@@ -350,6 +362,12 @@ int gmbe_ego_subgraph(const gmbe_graph* g, uint32_t u, uint64_t max_edges, gmbe_
     for (size_t i = 0; i < keepV.size(); ++i) (*out)->v_orig[i] = g->v_orig[keepV[i]];
   }
   return rc;
+}
+
+int gmbe_ego_subgraph(const gmbe_graph* g, uint32_t u, uint64_t max_edges, gmbe_graph** out) {
+  return gmbe::guarded("gmbe_ego_subgraph", [&] {
+    return gmbe_ego_subgraph_impl(g, u, max_edges, out);
+  });
 }
 
 }  // extern "C"

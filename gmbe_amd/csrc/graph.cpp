@@ -268,7 +268,7 @@ static std::vector<uint32_t> iota_u32(uint32_t n) {
   return r;
 }
 
-int gmbe_graph_read_adj(const char* path, const gmbe_option* opt, gmbe_graph** out) {
+static int gmbe_graph_read_adj_impl(const char* path, const gmbe_option* opt, gmbe_graph** out) {
   if (!path || !out) return set_error(GMBE_ERR_ARG, "null argument");
   *out = nullptr;
   FILE* fp = fopen(path, "rb");
@@ -327,8 +327,15 @@ int gmbe_graph_read_adj(const char* path, const gmbe_option* opt, gmbe_graph** o
   return GMBE_OK;
 }
 
-int gmbe_graph_from_csr(uint32_t nU, uint32_t nV, const int64_t* v_off, const uint32_t* v_adj,
-                        const gmbe_option* opt, gmbe_graph** out) {
+int gmbe_graph_read_adj(const char* path, const gmbe_option* opt, gmbe_graph** out) {
+  return gmbe::guarded("gmbe_graph_read_adj", [&] {
+    return gmbe_graph_read_adj_impl(path, opt, out);
+  });
+}
+
+static int gmbe_graph_from_csr_impl(uint32_t nU, uint32_t nV, const int64_t* v_off,
+                                    const uint32_t* v_adj, const gmbe_option* opt,
+                                    gmbe_graph** out) {
   if (!v_off || (!v_adj && nV && v_off[nV] > v_off[0]) || !out)
     return set_error(GMBE_ERR_ARG, "null argument");
   if (nU == UINT32_MAX || nV == UINT32_MAX)
@@ -347,7 +354,14 @@ int gmbe_graph_from_csr(uint32_t nU, uint32_t nV, const int64_t* v_off, const ui
   return GMBE_OK;
 }
 
-int gmbe_graph_derive(const gmbe_graph* src, const gmbe_option* opt, gmbe_graph** out) {
+int gmbe_graph_from_csr(uint32_t nU, uint32_t nV, const int64_t* v_off, const uint32_t* v_adj,
+                        const gmbe_option* opt, gmbe_graph** out) {
+  return gmbe::guarded("gmbe_graph_from_csr", [&] {
+    return gmbe_graph_from_csr_impl(nU, nV, v_off, v_adj, opt, out);
+  });
+}
+
+static int gmbe_graph_derive_impl(const gmbe_graph* src, const gmbe_option* opt, gmbe_graph** out) {
   if (!src || !out) return set_error(GMBE_ERR_ARG, "null argument");
   *out = nullptr;
   gmbe_option o;
@@ -364,7 +378,11 @@ int gmbe_graph_derive(const gmbe_graph* src, const gmbe_option* opt, gmbe_graph*
   return GMBE_OK;
 }
 
-int gmbe_graph_stats(const gmbe_graph* g, int64_t out[8]) {
+int gmbe_graph_derive(const gmbe_graph* src, const gmbe_option* opt, gmbe_graph** out) {
+  return gmbe::guarded("gmbe_graph_derive", [&] { return gmbe_graph_derive_impl(src, opt, out); });
+}
+
+static int gmbe_graph_stats_impl(const gmbe_graph* g, int64_t out[8]) {
   if (!g || !out) return set_error(GMBE_ERR_ARG, "null argument");
   out[0] = g->nU;
   out[1] = g->nV;
@@ -377,9 +395,13 @@ int gmbe_graph_stats(const gmbe_graph* g, int64_t out[8]) {
   return GMBE_OK;
 }
 
-int gmbe_graph_csr(const gmbe_graph* g, const int64_t** v_off, const uint32_t** v_adj,
-                   const int64_t** u_off, const uint32_t** u_adj, const uint32_t** u_orig,
-                   const uint32_t** v_orig) {
+int gmbe_graph_stats(const gmbe_graph* g, int64_t out[8]) {
+  return gmbe::guarded("gmbe_graph_stats", [&] { return gmbe_graph_stats_impl(g, out); });
+}
+
+static int gmbe_graph_csr_impl(const gmbe_graph* g, const int64_t** v_off, const uint32_t** v_adj,
+                               const int64_t** u_off, const uint32_t** u_adj, const uint32_t** u_orig,
+                               const uint32_t** v_orig) {
   if (!g) return set_error(GMBE_ERR_ARG, "null graph");
   if (v_off) *v_off = g->v_off.data();
   if (v_adj) *v_adj = g->v_adj.data();
@@ -390,7 +412,15 @@ int gmbe_graph_csr(const gmbe_graph* g, const int64_t** v_off, const uint32_t** 
   return GMBE_OK;
 }
 
-int gmbe_graph_write_adj(const gmbe_graph* g, const char* path) {
+int gmbe_graph_csr(const gmbe_graph* g, const int64_t** v_off, const uint32_t** v_adj,
+                   const int64_t** u_off, const uint32_t** u_adj, const uint32_t** u_orig,
+                   const uint32_t** v_orig) {
+  return gmbe::guarded("gmbe_graph_csr", [&] {
+    return gmbe_graph_csr_impl(g, v_off, v_adj, u_off, u_adj, u_orig, v_orig);
+  });
+}
+
+static int gmbe_graph_write_adj_impl(const gmbe_graph* g, const char* path) {
   if (!g || !path) return set_error(GMBE_ERR_ARG, "null argument");
   FILE* fp = fopen(path, "wb");
   if (!fp) return set_error(GMBE_ERR_IO, "can't open %s for writing", path);
@@ -411,7 +441,11 @@ int gmbe_graph_write_adj(const gmbe_graph* g, const char* path) {
   return GMBE_OK;
 }
 
-int gmbe_graph_write_edges(const gmbe_graph* g, const char* path) {
+int gmbe_graph_write_adj(const gmbe_graph* g, const char* path) {
+  return gmbe::guarded("gmbe_graph_write_adj", [&] { return gmbe_graph_write_adj_impl(g, path); });
+}
+
+static int gmbe_graph_write_edges_impl(const gmbe_graph* g, const char* path) {
   if (!g || !path) return set_error(GMBE_ERR_ARG, "null argument");
   // rows are the original V side, columns the original U side, in original ids
   const bool sw = g->swapped != 0;
@@ -448,6 +482,12 @@ int gmbe_graph_write_edges(const gmbe_graph* g, const char* path) {
   fclose(fp);
   if (!ok) return set_error(GMBE_ERR_IO, "write failed: %s", path);
   return GMBE_OK;
+}
+
+int gmbe_graph_write_edges(const gmbe_graph* g, const char* path) {
+  return gmbe::guarded("gmbe_graph_write_edges", [&] {
+    return gmbe_graph_write_edges_impl(g, path);
+  });
 }
 
 void gmbe_graph_destroy(gmbe_graph* g) { delete g; }

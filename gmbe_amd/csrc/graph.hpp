@@ -8,7 +8,11 @@
 #pragma once
 
 #include <cstdint>
+#include <exception>
+#include <new>
+#include <stdexcept>
 #include <string>
+#include <system_error>
 #include <vector>
 
 #include "../../include/gmbe.h"
@@ -32,6 +36,26 @@ namespace gmbe {
 
 // Thread-local last error (gmbe_last_error).
 int set_error(int code, const char* fmt, ...);
+
+// Runs the body of a C entry point.  A C++ exception (a host allocation, a
+// thread that cannot start) does not unwind into the C caller: it becomes an
+// error code, with its message behind gmbe_last_error().
+template <class F>
+int guarded(const char* fn, F&& body) noexcept {
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    return set_error(GMBE_ERR_NOMEM, "%s: host allocation failed", fn);
+  } catch (const std::length_error& e) {
+    return set_error(GMBE_ERR_CAPACITY, "%s: %s", fn, e.what());
+  } catch (const std::system_error& e) {
+    return set_error(GMBE_ERR_NOMEM, "%s: %s", fn, e.what());
+  } catch (const std::exception& e) {
+    return set_error(GMBE_ERR_INTERNAL, "%s: %s", fn, e.what());
+  } catch (...) {
+    return set_error(GMBE_ERR_INTERNAL, "%s: unknown exception", fn);
+  }
+}
 
 // Build a working graph from file-orientation rows.  rows[v] lists U ids.
 // v_orig_of_row / u_orig give the original ids of the input rows / U ids

@@ -247,8 +247,8 @@ int submit(gmbe_server* s, gmbe_graph* w, uint64_t* ticket) {
 
 extern "C" {
 
-int gmbe_serve_create(const gmbe_graph* resident, int slots, uint64_t mem_budget, int bound_height,
-                      int bound_size, gmbe_server** out) {
+static int gmbe_serve_create_impl(const gmbe_graph* resident, int slots, uint64_t mem_budget,
+                                  int bound_height, int bound_size, gmbe_server** out) {
   if (!out) return set_error(GMBE_ERR_ARG, "null argument");
   *out = nullptr;
   if (slots < 1 || slots > 65536) return set_error(GMBE_ERR_ARG, "slots must be in [1, 65536]");
@@ -296,7 +296,14 @@ int gmbe_serve_create(const gmbe_graph* resident, int slots, uint64_t mem_budget
   return GMBE_OK;
 }
 
-int gmbe_serve_set(gmbe_server* s, const char* key, int64_t value) {
+int gmbe_serve_create(const gmbe_graph* resident, int slots, uint64_t mem_budget, int bound_height,
+                      int bound_size, gmbe_server** out) {
+  return gmbe::guarded("gmbe_serve_create", [&] {
+    return gmbe_serve_create_impl(resident, slots, mem_budget, bound_height, bound_size, out);
+  });
+}
+
+static int gmbe_serve_set_impl(gmbe_server* s, const char* key, int64_t value) {
   if (!s || !key) return set_error(GMBE_ERR_ARG, "null argument");
   const std::string k(key);
   if (k == "request_ms") {
@@ -310,7 +317,11 @@ int gmbe_serve_set(gmbe_server* s, const char* key, int64_t value) {
   return set_error(GMBE_ERR_ARG, "unknown knob '%s'", key);
 }
 
-int gmbe_serve_stats(gmbe_server* s, uint64_t* out, int n) {
+int gmbe_serve_set(gmbe_server* s, const char* key, int64_t value) {
+  return gmbe::guarded("gmbe_serve_set", [&] { return gmbe_serve_set_impl(s, key, value); });
+}
+
+static int gmbe_serve_stats_impl(gmbe_server* s, uint64_t* out, int n) {
   if (!s || (n > 0 && !out)) return set_error(GMBE_ERR_ARG, "null argument");
   std::lock_guard<std::mutex> lk(s->mu);
   const uint64_t v[5] = {s->launches, s->launched, (uint64_t)(s->host_s * 1e6),
@@ -320,7 +331,11 @@ int gmbe_serve_stats(gmbe_server* s, uint64_t* out, int n) {
   return k;
 }
 
-int gmbe_serve_submit_graph(gmbe_server* s, const gmbe_graph* g, uint64_t* ticket) {
+int gmbe_serve_stats(gmbe_server* s, uint64_t* out, int n) {
+  return gmbe::guarded("gmbe_serve_stats", [&] { return gmbe_serve_stats_impl(s, out, n); });
+}
+
+static int gmbe_serve_submit_graph_impl(gmbe_server* s, const gmbe_graph* g, uint64_t* ticket) {
   if (!s || !g) return set_error(GMBE_ERR_ARG, "null argument");
   if (s->fatal.load()) return set_error(s->fatal.load(), "the server has failed");
   gmbe_option run{1, 1, 1};
@@ -330,7 +345,14 @@ int gmbe_serve_submit_graph(gmbe_server* s, const gmbe_graph* g, uint64_t* ticke
   return submit(s, w, ticket);
 }
 
-int gmbe_serve_submit_ego(gmbe_server* s, uint32_t u, uint64_t max_edges, uint64_t* ticket) {
+int gmbe_serve_submit_graph(gmbe_server* s, const gmbe_graph* g, uint64_t* ticket) {
+  return gmbe::guarded("gmbe_serve_submit_graph", [&] {
+    return gmbe_serve_submit_graph_impl(s, g, ticket);
+  });
+}
+
+static int gmbe_serve_submit_ego_impl(gmbe_server* s, uint32_t u, uint64_t max_edges,
+                                      uint64_t* ticket) {
   if (!s) return set_error(GMBE_ERR_ARG, "null argument");
   if (!s->resident) return set_error(GMBE_ERR_STATE, "the server has no resident graph");
   if (s->fatal.load()) return set_error(s->fatal.load(), "the server has failed");
@@ -345,7 +367,14 @@ int gmbe_serve_submit_ego(gmbe_server* s, uint32_t u, uint64_t max_edges, uint64
   return submit(s, w, ticket);
 }
 
-int gmbe_serve_poll(gmbe_server* s, gmbe_serve_done* out, int max, int wait, int* n_out) {
+int gmbe_serve_submit_ego(gmbe_server* s, uint32_t u, uint64_t max_edges, uint64_t* ticket) {
+  return gmbe::guarded("gmbe_serve_submit_ego", [&] {
+    return gmbe_serve_submit_ego_impl(s, u, max_edges, ticket);
+  });
+}
+
+static int gmbe_serve_poll_impl(gmbe_server* s, gmbe_serve_done* out, int max, int wait,
+                                int* n_out) {
   if (!s || !n_out || (max > 0 && !out)) return set_error(GMBE_ERR_ARG, "null argument");
   std::unique_lock<std::mutex> lk(s->mu);
   if (wait && s->done.empty() && s->outstanding > 0)
@@ -357,6 +386,12 @@ int gmbe_serve_poll(gmbe_server* s, gmbe_serve_done* out, int max, int wait, int
   }
   *n_out = n;
   return GMBE_OK;
+}
+
+int gmbe_serve_poll(gmbe_server* s, gmbe_serve_done* out, int max, int wait, int* n_out) {
+  return gmbe::guarded("gmbe_serve_poll", [&] {
+    return gmbe_serve_poll_impl(s, out, max, wait, n_out);
+  });
 }
 
 void gmbe_serve_destroy(gmbe_server* s) {

@@ -4,7 +4,9 @@
  * Every entry point takes plain pointers and sizes; no HIP or torch types
  * cross this boundary.  Return codes: 0 = ok, negative = error (see
  * GMBE_ERR_*); the message of the last error on the calling thread is
- * returned by gmbe_last_error().  The library never calls exit().
+ * returned by gmbe_last_error().  The library never calls exit(), and no
+ * C++ exception leaves an entry point: a failed host allocation returns
+ * GMBE_ERR_NOMEM, anything unexpected GMBE_ERR_INTERNAL.
  *
  * The reference's surface is a C++ class API used from src/main.cu:
  *   Option{uvtrans, order, fast_mode}                 src/BiGraph.h:13-22
@@ -40,6 +42,7 @@ extern "C" {
 #define GMBE_ERR_UNSUPPORTED (-8) /* option not supported on this build     */
 #define GMBE_ERR_STATE (-9)       /* call out of order (e.g. list before run) */
 #define GMBE_ERR_DEADLINE (-10)   /* deadline_ms passed; counts are partial  */
+#define GMBE_ERR_INTERNAL (-11)   /* unexpected host exception (a bug)       */
 
 typedef struct gmbe_graph gmbe_graph;
 typedef struct gmbe_finder gmbe_finder;

@@ -4,6 +4,8 @@
 // Host-only driver of the loader, writers and generators, built with
 // AddressSanitizer and UndefinedBehaviorSanitizer by tests/test_host_sanitizers.py.
 #include "gmbe.h"
+#include "../../gmbe_amd/csrc/graph.hpp"
+#include <stdexcept>
 #include <string>
 static int fails = 0;
 #define CHECK(x) do { if (!(x)) { printf("FAIL %s:%d %s\n", __FILE__, __LINE__, #x); ++fails; } } while (0)
@@ -47,6 +49,16 @@ int main(int argc, char** argv) {
   gmbe_graph* g = nullptr;
   CHECK(gmbe_generate(&p, &g) == 0);
   gmbe_graph_destroy(g);
+  // an exception inside an entry point's body becomes an error code and message
+  CHECK(gmbe::guarded("probe_a", []() -> int { throw std::bad_alloc(); }) == GMBE_ERR_NOMEM);
+  CHECK(strstr(gmbe_last_error(), "probe_a") != nullptr);
+  CHECK(gmbe::guarded("probe_b", []() -> int { throw std::length_error("too long"); }) ==
+        GMBE_ERR_CAPACITY);
+  CHECK(gmbe::guarded("probe_c", []() -> int { throw std::runtime_error("boom"); }) ==
+        GMBE_ERR_INTERNAL);
+  CHECK(strstr(gmbe_last_error(), "boom") != nullptr);
+  CHECK(gmbe::guarded("probe_d", []() -> int { throw 7; }) == GMBE_ERR_INTERNAL);
+  CHECK(gmbe::guarded("probe_e", []() -> int { return 5; }) == 5);
   printf("done, %d failures\n", fails);
   return fails != 0;
 }

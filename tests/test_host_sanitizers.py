@@ -1,7 +1,8 @@
 """The host-side loader, writers and generators under AddressSanitizer and
 UndefinedBehaviorSanitizer (g++, host code only): malformed and edge-case
 .adj files through every option combination, derive, both writers, the
-small-graph generator with ego cuts, and the planted-block generator."""
+small-graph generator with ego cuts, the planted-block generator, and the
+guard that turns an exception inside a C entry point into an error code."""
 import os
 import shutil
 import subprocess
