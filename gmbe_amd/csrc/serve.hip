@@ -291,7 +291,12 @@ static int gmbe_serve_create_impl(const gmbe_graph* resident, int slots, uint64_
     gmbe_serve_destroy(s);
     return rc;
   }
-  s->th = std::thread(server_loop, s);
+  try {
+    s->th = std::thread(server_loop, s);
+  } catch (...) {
+    gmbe_serve_destroy(s);  // no thread to join: releases the lanes only
+    throw;                  // reported as an error code by the entry point's guard
+  }
   *out = s;
   return GMBE_OK;
 }

@@ -314,7 +314,10 @@ int gmbe_serve_poll(gmbe_server* s, gmbe_serve_done* out, int max, int wait,
  * microseconds of launches on the GPU (events, summed over lanes), [4]
  * requests waiting.  Returns the number of values written. */
 int gmbe_serve_stats(gmbe_server* s, uint64_t* out, int n);
-/* Finishes the requests on the GPU, drops those still waiting. */
+/* Finishes the requests on the GPU, drops those still waiting.  Submit,
+ * poll, set and stats may be called from several threads at once; destroy
+ * is called once, after every other call on the server has returned (a
+ * poll still blocked in it would wait on freed state). */
 void gmbe_serve_destroy(gmbe_server* s);
 
 /* ---- misc ---------------------------------------------------------------- */
