@@ -23,7 +23,8 @@
  *   orc_imbea  restates the authors' improved CPU finder ImbeaAdvFinder
  *              (baselines/MBE.zip:MBE/BicliqueFinder.cpp:227-350): candidates
  *              ordered by overlap, dominated candidates moved to the excluded
- *              set with the pushed vertex; a further CPU baseline.
+ *              set with the pushed vertex (and out of P by the reference's
+ *              own merge walk); a further CPU baseline.
  *   orc_pmbe   restates the authors' CPU finder PmbeAdvFinder
  *              (baselines/MBE.zip:MBE/PMBE.cpp:3-77, 251-354): V renumbered
  *              by the neighbourhood-containment order, candidates skipped
@@ -782,8 +783,9 @@ static int64_t except(const uint32_t* a, int64_t na, const uint32_t* b, int64_t 
 
 /* ImbeaAdvFinder::biclique_find (baselines/MBE.zip:MBE/BicliqueFinder.cpp:
  * 282-350).  P is processed from its back; a candidate whose neighbourhood
- * holds L' and nothing of L \ L' is moved to Q with x after x's subtree.  P
- * and Q are this call's own arrays; Q has room for every entry of P. */
+ * holds L' and nothing of L \ L' joins Q with x after x's subtree, and
+ * leaves P as the reference's merge walk lets it (below).  P and Q are this
+ * call's own arrays; Q has room for every entry of P. */
 static int im_find(mb_ws* w, const uint32_t* L, int64_t nL, const uint32_t* R, int64_t nR,
                    uint64_t sR, uint32_t* P, int64_t nP, uint32_t* Q, int64_t nQ) {
   const og* g = w->g;
