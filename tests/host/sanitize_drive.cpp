@@ -1,7 +1,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-// Host-only driver of the loader, writers and generators, built with
+// Host-only test program of the loader, writers and generators, built with
 // AddressSanitizer and UndefinedBehaviorSanitizer by tests/test_host_sanitizers.py.
 #include "gmbe.h"
 #include "../../gmbe_amd/csrc/graph.hpp"

@@ -1,4 +1,4 @@
-/* Host-only driver of the CPU oracle (oracle/oracle.c), built with
+/* Host-only test program of the CPU oracle (oracle/oracle.c), built with
  * AddressSanitizer and UndefinedBehaviorSanitizer by
  * tests/test_host_sanitizers.py: the five enumerators on seeded random small
  * graphs must agree, with list output on, through their worker threads. */
