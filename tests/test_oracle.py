@@ -233,15 +233,18 @@ def test_root_sample_partition():
     assert x == full.hash_xor
 
 
-def test_mbea_time_limit_cuts_within_a_root():
-    """orc_mbea's time limit also abandons the subtree it is in: a tiny limit
-    returns promptly with a partial count, a generous one the full count."""
+@pytest.mark.parametrize("algo", ["mbea", "imbea", "pmbe"])
+def test_cpu_finder_time_limit_cuts_within_a_root(algo):
+    """The CPU finders' time limit also abandons the subtree they are in: a
+    tiny limit returns promptly with a partial count, a generous one the full
+    count (the benchmark's CPU legs rely on both)."""
+    run = getattr(oracle, algo)
     og = workloads.run_graph(2, 0.01).oracle_graph()
-    full = oracle.mbea(og, threads=8)
-    cut = oracle.mbea(og, threads=8, time_limit=0.01)
+    full = run(og, threads=8)
+    cut = run(og, threads=8, time_limit=0.01)
     assert cut.seconds < 2.0
     assert cut.maximal < full.maximal
-    again = oracle.mbea(og, threads=8, time_limit=600.0)
+    again = run(og, threads=8, time_limit=600.0)
     assert again.digest() == full.digest()
 
 
