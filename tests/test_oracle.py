@@ -220,11 +220,14 @@ def test_prune_keeps_the_result_and_saves_pushes():
     assert a.nodes <= b.nodes and a.non_maximal <= b.non_maximal
 
 
-def test_root_sample_partition():
-    """Digests over disjoint root samples add up to the full digest."""
+@pytest.mark.parametrize("algo", ["gmbe", "mbea", "imbea", "pmbe"])
+def test_root_sample_partition(algo):
+    """Digests over disjoint root samples add up to the full digest (PMBE's
+    ranks follow its own containment order)."""
+    run = getattr(oracle, algo)
     og = workloads.run_graph(2, 0.01).oracle_graph()
-    full = oracle.gmbe(og, threads=8)
-    parts = [oracle.gmbe(og, threads=8, stride=3, phase=p) for p in range(3)]
+    full = run(og, threads=8)
+    parts = [run(og, threads=8, stride=3, phase=p) for p in range(3)]
     assert sum(p.maximal for p in parts) == full.maximal
     assert sum(p.hash_sum for p in parts) % (1 << 64) == full.hash_sum
     x = 0
