@@ -1,7 +1,8 @@
 /* Host-only test program of the CPU oracle (oracle/oracle.c), built with
  * AddressSanitizer and UndefinedBehaviorSanitizer by
  * tests/test_host_sanitizers.py: the five enumerators on seeded random small
- * graphs must agree, with list output on, through their worker threads. */
+ * graphs (half of them with nested rows) must agree, with list output on,
+ * through their worker threads. */
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -47,8 +48,16 @@ int main(void) {
     off[0] = 0;
     for (uint32_t v = 0; v < nV; ++v) {
       int64_t k = off[v];
-      for (uint32_t u = 0; u < nU; ++u)
-        if (next_u32() % 100 < pct) adj[k++] = u;
+      if (t % 2 && v > 0 && next_u32() % 10 < 7) {
+        /* odd graphs: most rows are a subset of an earlier row (nested
+         * neighbourhoods, which PMBE's containment order and skips act on) */
+        uint32_t p = next_u32() % v;
+        for (int64_t e = off[p]; e < off[p + 1]; ++e)
+          if (next_u32() % 100 < 70) adj[k++] = adj[e];
+      } else {
+        for (uint32_t u = 0; u < nU; ++u)
+          if (next_u32() % 100 < pct) adj[k++] = u;
+      }
       off[v + 1] = k;
     }
     orc_result a, b, c, d, e, f;
