@@ -16,7 +16,9 @@ enumeration kernel (algorithmic bytes per launch / kernel time from HIP events
 on the finder's stream, against the 8 TB/s HBM peak) and the CPU baseline:
 the restatement of the reference's CPU finder (MbeaFinder with MbeaAdvFinder's
 2-hop top level) timed on this host's cores over a bounded sample of the same
-top-level vertices.
+top-level vertices.  Beside it, cpu_baseline_imbea and cpu_baseline_pmbe time
+the restatements of the authors' faster CPU finders (ImbeaAdvFinder and
+PmbeAdvFinder of baselines/MBE.zip) on samples of the same graph.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
     python bench.py --workload batch     # 4,096 small graphs: batched vs looped launches
