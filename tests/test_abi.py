@@ -22,6 +22,14 @@ def test_header_symbols_match_binding():
         "ctypes binding and include/gmbe.h disagree"
 
 
+def test_error_codes_match_header():
+    """Every GMBE_ERR_* the header defines has its name in the binding's map,
+    with the same value, and the map holds no other code."""
+    text = open(os.path.join(ROOT, "include", "gmbe.h")).read()
+    codes = {int(v): n for n, v in re.findall(r"#define (GMBE_ERR_\w+) \((-\d+)\)", text)}
+    assert codes and codes == _lib.ERRORS
+
+
 def test_library_exports_every_declared_symbol():
     l = C.CDLL(_lib.LIB_PATH)
     missing = [s for s in declared_symbols() if not hasattr(l, s)]
