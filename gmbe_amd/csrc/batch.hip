@@ -360,6 +360,10 @@ static int gmbe_batch_set_impl(gmbe_batch* b, const char* key, int64_t value) {
   else if (k == "group") { if ((rc = range(0, 1))) return rc; ip = &kn.group; }
   else if (k == "share") { if ((rc = range(0, 1))) return rc; ip = &kn.share; }
   else if (k == "split") { if ((rc = range(0, 1))) return rc; ip = &kn.split; }
+  else if (k == "dom_k") { if ((rc = range(0, mbe::kMaxDom))) return rc; ip = &kn.dom_k; }
+  else if (k == "dom_skip") { if ((rc = range(0, 1))) return rc; ip = &kn.dom_skip; }
+  else if (k == "recompute") { if ((rc = range(0, 1))) return rc; ip = &kn.recompute; }
+  else if (k == "count_bytes") { if ((rc = range(0, 1))) return rc; ip = &kn.count_bytes; }
   else if (k == "share_work") { if ((rc = range(1, 1ll << 40))) return rc; slot = &kn.share_work; }
   else if (k == "merge_min") { if ((rc = range(2, 1ll << 30))) return rc; slot = &kn.merge_min; }
   else if (k == "merge_branch") { if ((rc = range(0, 1ll << 30))) return rc; slot = &kn.merge_branch; }

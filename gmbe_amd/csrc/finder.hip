@@ -29,10 +29,11 @@ void mbe_persistent(Params P) {
   __shared__ __attribute__((aligned(16))) char lds[WAVES * (LDS_PER_WAVE > 0 ? LDS_PER_WAVE : 16)];
   __shared__ uint32_t s_off[WAVES][kWave + 1];
   __shared__ int64_t s_start[WAVES][kWave];
+  __shared__ Frame s_fr[WAVES][kLdsFrames];
   const uint32_t wv = threadIdx.x / kWave;
   const uint64_t gw = (uint64_t)blockIdx.x * WAVES + wv;
   WaveMem m = wave_mem(P, gw, lds + wv * (LDS_PER_WAVE > 0 ? LDS_PER_WAVE : 16), LDS_PER_WAVE,
-                       s_off[wv], s_start[wv]);
+                       s_off[wv], s_start[wv], s_fr[wv]);
   Counters c;
   Task t;
   int k = 0;
@@ -138,6 +139,11 @@ void mbe_persistent(Params P) {
                            __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_fetch_add(a + A_LATE, (unsigned long long)c.late, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t red[3] = {c.c_reduce, c.dropped, c.skipped};
+    for (int i = 0; i < 3; ++i)
+      if (red[i])
+        __hip_atomic_fetch_add(a + A_CYC_REDUCE + i, (unsigned long long)red[i], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -220,6 +226,7 @@ KernelChoice pick_kernel(int lds_kb, int waves, bool wide) {
   if (waves == 2 && lds_kb == 16) return {mbe_persistent<2, 16384, kMainW, 2>, 2, 16384};
   if (waves != 4) return {nullptr, 0, 0};
   switch (lds_kb) {
+    case 7: return {mbe_persistent<4, 7168, kMainW, 4>, 4, 7168};
     case 0: return {mbe_persistent<4, 0, kMainW, 2>, 4, 0};
     case 4: return {mbe_persistent<4, 4096, kMainW, 2>, 4, 4096};
     case 8: return {mbe_persistent<4, 8192, kMainW, 2>, 4, 8192};
@@ -380,6 +387,7 @@ mbe::Scratch scratch_layout(uint32_t nV, uint32_t dmaxV, uint64_t arena_bytes,
   s.o_kC = take(nv * 8);
   s.o_R = take(nv * 4);
   s.o_fr = take((d + 2) * sizeof(mbe::Frame));
+  s.o_dom = take((uint64_t)mbe::kMaxDom * mbe::kMaxW * 8);
   s.o_arena = take(arena_bytes);
   s.arena_bytes = arena_bytes;
   s.o_ctx = take(ctx_bytes * ctx_slots);
@@ -402,6 +410,9 @@ struct gmbe_finder {
   int64_t root_begin = 0, root_end = -1;  // processing-order rank range, -1 = all
   int64_t deadline_ms = 0;                 // 0 = run to completion
   int64_t late_share = 32;                 // 0 = off
+  int64_t dom_k = 8, dom_skip = 1;         // list reduction (reduce_list)
+  int64_t recompute = 0;                   // split large tasks without a free context into children
+  int64_t count_bytes = 0;                 // count algorithmic bytes (alg_bytes)
   int64_t ctx_slots = 8;                   // shared-context slots per wave
   int64_t profile = 0, group = 1, share = 1, ctx_kb = 4096, share_work = 4096, share_queue = -1;
   int64_t merge_min = 256, merge_branch = 64;
@@ -556,8 +567,8 @@ static int gmbe_finder_set_impl(gmbe_finder* f, const char* key, int64_t value) 
   else if (k == "prune") { if ((rc = range(0, 1))) return rc; f->prune = value; }
   else if (k == "max_prefix") { if ((rc = range(2, mbe::kMaxPrefix))) return rc; f->max_prefix = value; }
   else if (k == "lds_kb") {
-    if (value != 0 && value != 4 && value != 8 && value != 12 && value != 16 && value != 24)
-      return set_error(GMBE_ERR_ARG, "lds_kb must be 0, 4, 8, 12, 16 or 24");
+    if (value != 0 && value != 4 && value != 7 && value != 8 && value != 12 && value != 16 && value != 24)
+      return set_error(GMBE_ERR_ARG, "lds_kb must be 0, 4, 7, 8, 12, 16 or 24");
     f->lds_kb = value;
   }
   else if (k == "queue_log2") { if ((rc = range(4, 30))) return rc; f->queue_log2 = value; }
@@ -575,6 +586,10 @@ static int gmbe_finder_set_impl(gmbe_finder* f, const char* key, int64_t value) 
   else if (k == "ctx_slots") { if ((rc = range(1, 32))) return rc; f->ctx_slots = value; }
   else if (k == "exit_stamps") { if ((rc = range(0, 1))) return rc; f->exit_stamps = value; }
   else if (k == "late_share") { if ((rc = range(0, 1ll << 30))) return rc; f->late_share = value; }
+  else if (k == "dom_k") { if ((rc = range(0, mbe::kMaxDom))) return rc; f->dom_k = value; }
+  else if (k == "dom_skip") { if ((rc = range(0, 1))) return rc; f->dom_skip = value; }
+  else if (k == "recompute") { if ((rc = range(0, 1))) return rc; f->recompute = value; }
+  else if (k == "count_bytes") { if ((rc = range(0, 1))) return rc; f->count_bytes = value; }
   else if (k == "deadline_ms") { if ((rc = range(0, 1ll << 32))) return rc; f->deadline_ms = value; }
   else if (k == "root_begin") { if ((rc = range(0, (1ll << 32)))) return rc; f->root_begin = value; }
   else if (k == "root_end") { if ((rc = range(-1, (1ll << 32)))) return rc; f->root_end = value; }
@@ -660,6 +675,10 @@ static int gmbe_finder_execute_impl(gmbe_finder* f, gmbe_result* res) {
     P.exit_stamps = f->stamps;
   }
   P.late_share = f->share && f->sel == 2 ? (int)f->late_share : 0;  // a form of splitting
+  P.dom_k = (int)f->dom_k;
+  P.dom_skip = (int)f->dom_skip;
+  P.recompute = (int)f->recompute;
+  P.count_bytes = (int)f->count_bytes;
   P.share_work = (int)std::min<int64_t>(f->share_work, INT32_MAX);
   P.share_queue = f->share_queue >= 0 ? (int)f->share_queue : f->nwaves;
   KernelChoice kc = pick_kernel((int)f->lds_kb, 4, false);

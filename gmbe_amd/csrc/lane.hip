@@ -365,6 +365,10 @@ mbe::Params Lane::params(bool wide) const {
   P.share_work = (int)std::min<int64_t>(knobs.share_work, INT32_MAX);
   P.share_queue = knobs.share_queue >= 0 ? (int)knobs.share_queue : nwaves;
   P.late_share = P.share ? (int)knobs.late_share : 0;
+  P.dom_k = knobs.dom_k;
+  P.dom_skip = knobs.dom_skip;
+  P.recompute = knobs.recompute;
+  P.count_bytes = knobs.count_bytes;
   P.budget = (uint64_t)knobs.budget_ms * (uint64_t)eng->wall_khz;
   return P;
 }

@@ -45,6 +45,11 @@ constexpr int kShards = 32;                // independent queue rings
 constexpr int kCtlStride = 32;             // u64 between control counters (256 B)
 constexpr int kMaxW = 16;                  // widest bitset (|L| <= 1024)
 constexpr int kMainW = 4;                  // widest bitset of the main kernel
+constexpr int kMaxDom = 16;                // dominators tried per list (reduce_list)
+constexpr uint32_t kLdsFrames = 32;        // DFS frames kept in LDS (deeper ones in the slab)
+// flag in an entry's minidx: the entry is not pushed at this node (its push
+// would be non-maximal); children copy minidx without it
+constexpr uint32_t kSkip = 0x80000000u;
 
 struct Task {
   uint32_t v[kMaxPrefix];
@@ -82,7 +87,7 @@ enum Acc : int {
   A_CYC_WAIT, A_CYC_PREFIX, A_CYC_EXPAND, A_CYC_EXTRACT, A_CYC_BUILD, A_CYC_DFS, A_CYC_SPLIT,
   A_EDGES, A_ENTRIES, A_OVF_ARENA, A_OVF_WIDE, A_OVF_DEFER, A_ENT_W1, A_NODES_W1, A_CYC_TAIL,
   A_CYC_PASS, A_CYC_MERGE, A_CYC_COMPRESS, A_CYC_PUBLISH, A_PUBLISHES, A_MERGES, A_ABANDONED,
-  A_LATE, A_COUNT
+  A_LATE, A_CYC_REDUCE, A_DROPPED, A_SKIPPED, A_COUNT
 };
 
 struct Sched {
@@ -110,7 +115,7 @@ struct Scratch {
   uint64_t slab;     // bytes per wave
   // offsets inside a slab
   uint64_t o_cmap, o_bm0, o_bm1, o_L, o_Lp, o_T, o_kL, o_U, o_Ucnt, o_C, o_kC, o_R, o_fr,
-      o_arena, o_ctx;
+      o_dom, o_arena, o_ctx;
   uint64_t arena_bytes, ctx_bytes;  // ctx_bytes: per context slot
   uint32_t ctx_slots;                // context slots per wave
   uint32_t nW0, nW1;
@@ -134,6 +139,10 @@ struct Params {
   int merge_min;             // merge twins of child lists at least this long
   int merge_branch;          // ... with at least this many pushable entries
   int late_share;            // min list length of a frame handed over late (0 = off)
+  int dom_k;                 // dominators tried when a list is reduced (0 = no reduction)
+  int dom_skip;              // mark pushes a dominator shows non-maximal (reduce_list)
+  int recompute;             // a large task with no free context slot: split into recomputed children
+  int count_bytes;           // count the algorithmic bytes of every node (alg_bytes)
   uint64_t budget;           // wall-clock ticks a wave may run (0 = no deadline)
   uint64_t deadline;         // set per wave: its start + budget
   unsigned long long* exit_stamps;  // if set: wall clock at which each block's waves were done
@@ -150,6 +159,7 @@ struct Counters {
   uint64_t c_pass = 0, c_merge = 0, c_compress = 0, c_publish = 0, publishes = 0, merges = 0;
   uint64_t abandoned = 0;              // subtrees left unfinished at the deadline
   uint64_t late = 0;                   // frames handed over by late sharing
+  uint64_t c_reduce = 0, dropped = 0, skipped = 0;  // list reduction: cycles, entries
 };
 
 // Per-graph result counters of a batch (u64 each, kPerGraph per graph).
@@ -524,7 +534,9 @@ struct WaveMem {
   uint32_t* C;     // candidates
   uint64_t* kC;    // hash keys of C
   uint32_t* R;     // closure of L
-  Frame* fr;       // DFS frames
+  Frame* fr;       // DFS frames (slab; the first kLdsFrames live in fr_lds)
+  Frame* fr_lds;   // DFS frames in LDS
+  uint64_t* dom;   // dominator bitsets of reduce_list (kMaxDom x kMaxW)
   char* arena;     // HBM list arena
   uint64_t arena_bytes;
   char* lds;       // LDS list arena
@@ -534,13 +546,34 @@ struct WaveMem {
   const DevGraph* g;  // graph of the current unit of work (set per unit)
 };
 
+// Frame i of the wave's DFS stack: LDS for the shallow ones.
+__device__ __forceinline__ void frame_put(const WaveMem& m, uint32_t i, const Frame& F) {
+  if (lane_id() == 0) {
+    if (i < kLdsFrames) m.fr_lds[i] = F;
+    else m.fr[i] = F;
+  }
+  wave_mem_wait();
+}
+__device__ __forceinline__ Frame frame_get(const WaveMem& m, uint32_t i) {
+  Frame F = i < kLdsFrames ? m.fr_lds[i] : m.fr[i];
+  F.sumR = uni64(F.sumR);
+  F.rec = uni64(F.rec);
+  F.len = uni(F.len);
+  F.cur = uni(F.cur);
+  F.curidx = uni(F.curidx);
+  F.nbits = uni(F.nbits);
+  F.tier = uni(F.tier);
+  F.top = uni(F.top);
+  return F;
+}
+
 __device__ __forceinline__ uint32_t graph_id(const Params& P, const WaveMem& m) {
   return (uint32_t)(m.g - P.gt);
 }
 
 __device__ __forceinline__ WaveMem wave_mem(const Params& P, uint64_t gw, char* lds,
                                             uint32_t lds_bytes, uint32_t* s_off,
-                                            int64_t* s_start) {
+                                            int64_t* s_start, Frame* fr_lds) {
   const Scratch& S = P.w;
   char* b = S.base + gw * S.slab;
   WaveMem m;
@@ -557,6 +590,8 @@ __device__ __forceinline__ WaveMem wave_mem(const Params& P, uint64_t gw, char* 
   m.kC = (uint64_t*)(b + S.o_kC);
   m.R = (uint32_t*)(b + S.o_R);
   m.fr = (Frame*)(b + S.o_fr);
+  m.fr_lds = fr_lds;
+  m.dom = (uint64_t*)(b + S.o_dom);
   m.arena = b + S.o_arena;
   m.arena_bytes = S.arena_bytes;
   m.lds = lds;
@@ -674,9 +709,9 @@ struct PassOut {
 template <int W, bool LIST_REMOTE>
 __device__ PassOut push_pass(Rec<W>* recs, uint32_t len, const uint64_t* ax, uint32_t xidx,
                              Rec<W>* crec, bool prune) {
-  // U chunks of 64 entries per iteration: all their loads are issued before
-  // the first is used, so that a wave keeps several list reads in flight
-  constexpr int U = W == 1 ? 4 : (W == 2 ? 2 : 1);
+  // U chunks of 64 entries per iteration (their loads are issued before the
+  // first is used); reduced lists are mostly a single chunk, so one
+  constexpr int U = 1;
   const uint32_t lane = lane_id();
   bool nonmax = false;
   // once the node is known to be non-maximal its child list is never used:
@@ -713,8 +748,8 @@ __device__ PassOut push_pass(Rec<W>* recs, uint32_t len, const uint64_t* ax, uin
 #pragma unroll
       for (int w = 0; w < W; ++w) sub[u] = sub[u] && ((ae[u][w] & ax[w]) == ax[w]);
       if (sub[u]) {
-        if (mi[u] < xidx) nonmax = true;  // a twin or superset before x
-        else absorbed += ks[u];           // joins R'
+        if ((mi[u] & ~kSkip) < xidx) nonmax = true;  // a twin or superset before x
+        else absorbed += ks[u];                      // joins R'
       }
     }
     dead = dead || ballot(nonmax) != 0;
@@ -741,7 +776,7 @@ __device__ PassOut push_pass(Rec<W>* recs, uint32_t len, const uint64_t* ax, uin
       const uint64_t km = ballot(keep);
       const uint32_t pos = clen + lanes_below(km);
       if (keep) {
-        crec[pos].minidx = mi[u];
+        crec[pos].minidx = mi[u] & ~kSkip;  // a skip mark holds at its own node only
         crec[pos].rep = crep;
         crec[pos].ksum = ks[u];
 #pragma unroll
@@ -751,7 +786,7 @@ __device__ PassOut push_pass(Rec<W>* recs, uint32_t len, const uint64_t* ax, uin
           best_child = c < best_child ? c : best_child;
         }
       }
-      if (valid && br && !pruned) {
+      if (valid && br && !pruned && !(mi[u] & kSkip)) {
         uint64_t c = ((uint64_t)rep << 32) | k;
         best_next = c < best_next ? c : best_next;
       }
@@ -762,9 +797,9 @@ __device__ PassOut push_pass(Rec<W>* recs, uint32_t len, const uint64_t* ax, uin
   PassOut o;
   o.cbranch = cbranch;
   o.nonmax = ballot(nonmax) != 0;
-  o.absorbed = uni64(wave_sum64(absorbed));
-  o.best_next = uni64(wave_min64(best_next));
-  o.best_child = uni64(wave_min64(best_child));
+  o.absorbed = wave_sum_u64(absorbed);
+  o.best_next = (wave_min_pair(best_next));
+  o.best_child = (wave_min_pair(best_child));
   o.clen = clen;
   return o;
 }
@@ -776,7 +811,7 @@ __device__ __forceinline__ uint64_t key_sum_of_bits(const uint64_t* ax, const Ta
 #pragma unroll
   for (int w = 0; w < W; ++w)
     if ((ax[w] >> lane) & 1ull) sl += rd<KL_R>(ta.kL + w * 64 + lane);
-  return uni64(wave_sum64(sl));
+  return wave_sum_u64(sl);
 }
 
 // Re-index the bitsets of a list onto the set bits of ax (at most 64): entry
@@ -798,7 +833,7 @@ __device__ void compress_list(const Rec<W>* src, uint32_t len, const uint64_t* a
         ++t;
       }
     }
-    dst[i].minidx = src[i].minidx;
+    dst[i].minidx = src[i].minidx & ~kSkip;
     dst[i].rep = src[i].rep;
     dst[i].ksum = src[i].ksum;
     dst[i].a[0] = out;
@@ -887,7 +922,7 @@ __device__ bool publish_list(const Params& P, uint64_t gw, uint32_t ctx, uint32_
     bool it = false;
     if (i < len) {
       const uint32_t r = root[i].rep;
-      it = r != kNone && r >= min_rep;
+      it = r != kNone && r >= min_rep && !(root[i].minidx & kSkip);
     }
     n_items += __popcll(ballot(it));
   }
@@ -895,14 +930,16 @@ __device__ bool publish_list(const Params& P, uint64_t gw, uint32_t ctx, uint32_
   const uint32_t ne = (n_items + kItemsPerEntry - 1) / kItemsPerEntry;
   // copy the context with device-scope stores
   for (uint32_t i = lane; i < len; i += kWave) {
-    st_dev(&recs[i].minidx, root[i].minidx);
+    st_dev(&recs[i].minidx, root[i].minidx & ~kSkip);
     st_dev(&recs[i].rep, root[i].rep);
     st_dev(&recs[i].ksum, root[i].ksum);
 #pragma unroll
     for (int w = 0; w < W; ++w) st_dev(&recs[i].a[w], root[i].a[w]);
   }
-  for (uint32_t i = lane; i < nC; i += kWave)
-    st_dev((uint32_t*)ta.C + i, rd<C_R>(src.C + i));
+  // candidate ids: read below only to count algorithmic bytes
+  if (P.count_bytes)
+    for (uint32_t i = lane; i < nC; i += kWave)
+      st_dev((uint32_t*)ta.C + i, rd<C_R>(src.C + i));
   for (uint32_t j = lane; j < nL; j += kWave)
     st_dev((uint64_t*)ta.kL + j, rd<KL_R>(src.kL + j));
   if (lane == 0) {
@@ -934,7 +971,7 @@ __device__ bool publish_list(const Params& P, uint64_t gw, uint32_t ctx, uint32_
     bool it = false;
     if (i < len) {
       const uint32_t r = root[i].rep;
-      it = r != kNone && r >= min_rep;
+      it = r != kNone && r >= min_rep && !(root[i].minidx & kSkip);
     }
     uint64_t bm = ballot(it);
     if (it) {
@@ -969,45 +1006,42 @@ __device__ bool publish_list(const Params& P, uint64_t gw, uint32_t ctx, uint32_
   return true;
 }
 
-// The stack machine from frame F (a list in this wave's arenas).
 // Merge the entries of a list (len entries at crec) whose bitsets are
-// identical: minidx and rep by minimum, ksum by sum.  Open addressing in a
-// table taken from the arenas (LDS first); a slot's identity word packs a
-// 32-bit hash of the bitset with the index of its first entry, set by one CAS,
-// and a probe that meets the same hash compares the full bitsets, so no lane
-// ever waits for another.  Returns the new length and recomputes the first
-// push and the pushable count; leaves the list unchanged (returns len) when
-// the arenas lack room.
+// identical: minidx and rep by minimum, ksum by sum, in place.  Open
+// addressing in a table taken from the arenas (LDS first); a slot's identity
+// word packs a 32-bit hash of the bitset with the index of its first-inserted
+// entry, set by one CAS, and a probe that meets the same hash compares the
+// full bitsets, so no lane ever waits for another.  Each entry then parks its
+// slot number in its minidx (already folded into the slot), and a stable
+// compaction keeps the smallest entry of every slot with the merged fields:
+// it writes only at or below the position it reads, and never reads another
+// entry.  Returns the new length and recomputes the first push and the
+// pushable count; leaves the list unchanged (returns len) when the arenas
+// lack room for the table.
 template <int W>
 __device__ uint32_t merge_twins(const WaveMem& m, Rec<W>* crec, uint32_t len, uint32_t& lds_top,
                                 uint64_t& hbm_top, uint64_t& best_child, uint32_t& cbranch) {
   const uint32_t lane = lane_id();
   struct Slot {
-    unsigned long long id;  // hash << 32 | (first entry + 1); 0 = empty
+    unsigned long long id;  // hash << 32 | (first-inserted entry + 1); 0 = empty
     uint32_t mi, rep;
     uint64_t ksum;
+    uint32_t first;         // smallest entry of the slot
+    uint32_t pad;
   };
   uint32_t T = 64;
   while (T < 2 * len) T <<= 1;
-  const uint32_t save_l = lds_top;
-  const uint64_t save_h = hbm_top;
-  uint32_t tier = 1, otier = 1;
-  uint64_t tb = 0, ob = 0;
-  uint64_t taddr = arena_take(m, lds_top, hbm_top, (uint64_t)T * sizeof(Slot), tier, tb);
+  uint32_t tier = 1;
+  uint64_t tb = 0;
+  const uint64_t taddr = arena_take(m, lds_top, hbm_top, (uint64_t)T * sizeof(Slot), tier, tb);
   if (taddr == 0) return len;
-  if (tier == 0) lds_top = (uint32_t)(tb + (uint64_t)T * sizeof(Slot));
-  else hbm_top = tb + (uint64_t)T * sizeof(Slot);
-  uint64_t oaddr = arena_take(m, lds_top, hbm_top, (uint64_t)len * sizeof(Rec<W>), otier, ob);
-  lds_top = save_l;
-  hbm_top = save_h;
-  if (oaddr == 0) return len;
   Slot* tab = (Slot*)taddr;
-  Rec<W>* out = (Rec<W>*)oaddr;
   for (uint32_t i = lane; i < T; i += kWave) {
     tab[i].id = 0;
     tab[i].mi = kNone;
     tab[i].rep = kNone;
     tab[i].ksum = 0;
+    tab[i].first = kNone;
   }
   wave_mem_wait();
   for (uint32_t i = lane; i < len; i += kWave) {
@@ -1034,43 +1068,251 @@ __device__ uint32_t merge_twins(const WaveMem& m, Rec<W>* crec, uint32_t len, ui
       }
       h = (h + 1) & (T - 1);
     }
-    __hip_atomic_fetch_min(&tab[h].mi, crec[i].minidx, __ATOMIC_RELAXED,
+    __hip_atomic_fetch_min(&tab[h].mi, crec[i].minidx & ~kSkip, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_WORKGROUP);
     const uint32_t rp = crec[i].rep;
     if (rp != kNone)
       __hip_atomic_fetch_min(&tab[h].rep, rp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     add_wg(&tab[h].ksum, crec[i].ksum);
+    __hip_atomic_fetch_min(&tab[h].first, i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    // every probe that compares against this entry's bitset has its answer
+    // from the CAS above; the bitset itself is left untouched
+    st_wg(&crec[i].minidx, h);
   }
   wave_mem_wait();
   uint32_t n = 0, nbr = 0;
   uint64_t best = ~0ull;
-  for (uint32_t base = 0; base < T; base += kWave) {
+  for (uint32_t base = 0; base < len; base += kWave) {
     const uint32_t i = base + lane;
-    const unsigned long long id = i < T ? ld_wg(&tab[i].id) : 0ull;
-    const uint64_t bm = ballot(id != 0);
-    uint32_t rp = kNone;
-    if (id != 0) {
-      const uint32_t pos = n + lanes_below(bm), f = (uint32_t)(id & 0xffffffffu) - 1u;
-      rp = ld_wg(&tab[i].rep);
-      out[pos].minidx = ld_wg(&tab[i].mi);
-      out[pos].rep = rp;
-      out[pos].ksum = ld_wg(&tab[i].ksum);
+    bool keep = false;
+    uint32_t h = 0;
+    uint64_t a[W];
 #pragma unroll
-      for (int w = 0; w < W; ++w) out[pos].a[w] = crec[f].a[w];
+    for (int w = 0; w < W; ++w) a[w] = 0;
+    if (i < len) {
+      h = ld_wg(&crec[i].minidx);
+      keep = ld_wg(&tab[h].first) == i;
+#pragma unroll
+      for (int w = 0; w < W; ++w) a[w] = crec[i].a[w];
+    }
+    const uint64_t km = ballot(keep);
+    const uint32_t pos = n + lanes_below(km);
+    uint32_t rp = kNone;
+    wave_mem_wait();  // every lane has read its entry before any is overwritten
+    if (keep) {
+      rp = ld_wg(&tab[h].rep);
+      crec[pos].minidx = ld_wg(&tab[h].mi);
+      crec[pos].rep = rp;
+      crec[pos].ksum = ld_wg(&tab[h].ksum);
+#pragma unroll
+      for (int w = 0; w < W; ++w) crec[pos].a[w] = a[w];
       if (rp != kNone) {
         const uint64_t c = ((uint64_t)rp << 32) | pos;
         best = c < best ? c : best;
       }
     }
     nbr += __popcll(ballot(rp != kNone));
-    n += __popcll(bm);
+    n += __popcll(km);
   }
   wave_mem_wait();
-  for (uint32_t i = lane; i < n; i += kWave) crec[i] = out[i];
-  wave_mem_wait();
-  best_child = uni64(wave_min64(best));
+  best_child = (wave_min_pair(best));
   cbranch = nbr;
   return n;
+}
+
+// Reduce a node's list before its first push.  With minrep the smallest
+// pushable candidate of the list, an entry with no pushable member and
+// minidx < minrep (an excluded entry) precedes every push made anywhere in
+// the node's subtree and is never pushed or absorbed there: its only effect
+// is the non-maximal verdict of a push whose L' lies inside its bitset
+// (finder_push's maximality test, src/IterFinderGpu.h:252-261).  Hence
+//   * an excluded entry whose bitset lies inside another excluded entry's
+//     decides no verdict the other does not, in this node or below (both
+//     are restricted to the same L' at every push): it is dropped;
+//   * a pushable entry whose bitset lies inside an excluded entry's, or whose
+//     own group has a member before its pushable one, is non-maximal whenever
+//     this node pushes it: with dom_skip it is marked kSkip and not pushed
+//     here (the mark is not inherited: below an earlier push it may be
+//     maximal).
+// The excluded entries tried as dominators are the K of largest popcount
+// (ties to the earlier position); an entry equal to a dominator is dropped
+// only when it comes later.  The list is compacted in place, in order.
+// Returns the new length, the first push (rep << 32 | position) and the
+// number of pushable entries not marked.
+// One chunk (len <= 64): entries stay in registers, each dominator's bitset
+// is read from its lane, and the dominators are chosen and applied round by
+// round (a dropped entry is never chosen, and a chosen one is never dropped
+// later: a later dominator has no more bits and, if equal, a later lane).
+template <int W>
+__device__ uint32_t reduce_chunk(const Params& P, Rec<W>* recs, uint32_t len, int K,
+                                 uint32_t minrep, uint64_t& best, uint32_t& cbranch,
+                                 Counters& ctr) {
+  const uint32_t lane = lane_id();
+  const bool valid = lane < len;
+  uint32_t mi = kNone, rp = kNone;
+  uint64_t ks = 0, a[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) a[w] = 0;
+  if (valid) {
+    mi = recs[lane].minidx & ~kSkip;
+    rp = recs[lane].rep;
+    ks = recs[lane].ksum;
+#pragma unroll
+    for (int w = 0; w < W; ++w) a[w] = recs[lane].a[w];
+  }
+  const bool excl = valid && rp == kNone && mi < minrep;
+  const bool push = valid && rp != kNone;
+  uint32_t key = 0;
+  if (excl) {
+    uint32_t pc = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) pc += __popcll(a[w]);
+    key = ((pc + 1) << 6) | (63u - lane);
+  }
+  bool drop = false, sk = push && mi < rp;
+  for (int r = 0; r < K; ++r) {
+    const uint32_t mx = wave_max_u32(drop ? 0u : key);
+    if (mx == 0) break;
+    const uint32_t pd = 63u - (mx & 63u);
+    bool inside = true, eq = true;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const uint64_t dw = read_lane64(a[w], pd);
+      inside = inside && (a[w] & ~dw) == 0;
+      eq = eq && a[w] == dw;
+    }
+    if (inside) {
+      drop = drop || (excl && lane != pd && !(eq && lane < pd));
+      sk = sk || push;
+    }
+    if (lane == pd) key = 0;
+  }
+  if (!P.dom_skip) sk = false;
+  const bool keep = valid && !drop;
+  const uint64_t km = ballot(keep);
+  const uint32_t pos = lanes_below(km);
+  wave_mem_wait();  // every lane has read its entry before any is overwritten
+  uint64_t c = ~0ull;
+  if (keep) {
+    recs[pos].minidx = mi | (sk ? kSkip : 0u);
+    recs[pos].rep = rp;
+    recs[pos].ksum = ks;
+#pragma unroll
+    for (int w = 0; w < W; ++w) recs[pos].a[w] = a[w];
+    if (push && !sk) c = ((uint64_t)rp << 32) | pos;
+  }
+  wave_mem_wait();
+  best = wave_min_pair(c);
+  cbranch = __popcll(ballot(keep && push && !sk));
+  ctr.dropped += __popcll(ballot(valid && drop));
+  ctr.skipped += __popcll(ballot(keep && sk));
+  return __popcll(km);
+}
+
+template <int W>
+__device__ uint32_t reduce_list(const Params& P, const WaveMem& m, Rec<W>* recs, uint32_t len,
+                                uint64_t& best, uint32_t& cbranch, Counters& ctr) {
+  const uint32_t lane = lane_id();
+  const int K = P.dom_k < kMaxDom ? P.dom_k : kMaxDom;
+  if (K <= 0 || best == ~0ull || len < 2 || len >= (1u << 20)) return len;
+  const uint64_t tr = P.profile ? cycles() : 0;
+  if (len <= (uint32_t)kWave) {
+    const uint32_t n = reduce_chunk<W>(P, recs, len, K, (uint32_t)(best >> 32), best, cbranch, ctr);
+    if (P.profile) ctr.c_reduce += cycles() - tr;
+    return n;
+  }
+  const uint32_t minrep = (uint32_t)(best >> 32);
+  // dominators: K rounds of a maximum over key = (popcount + 1) << 20 | ~pos
+  // among the excluded entries, each round below the previous maximum
+  uint32_t dpos[kMaxDom];
+  int nd = 0;
+  uint32_t prev = 0xFFFFFFFFu;
+  for (int r = 0; r < K; ++r) {
+    uint32_t kb = 0;
+    for (uint32_t base = 0; base < len; base += kWave) {
+      const uint32_t i = base + lane;
+      if (i < len && recs[i].rep == kNone && (recs[i].minidx & ~kSkip) < minrep) {
+        uint32_t pc = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) pc += __popcll(recs[i].a[w]);
+        const uint32_t k = ((pc + 1) << 20) | (0xFFFFFu - i);
+        if (k < prev && k > kb) kb = k;
+      }
+    }
+    const uint32_t mx = wave_max_u32(kb);
+    if (mx == 0) break;
+    const uint32_t pd = 0xFFFFFu - (mx & 0xFFFFFu);
+    // keep the dominator's bitset: compaction may overwrite its entry
+    if (lane < (uint32_t)W) m.dom[nd * kMaxW + lane] = recs[pd].a[lane];
+    dpos[nd++] = pd;
+    prev = mx;
+  }
+  wave_mem_wait();
+  if (nd == 0 && !P.dom_skip) {
+    if (P.profile) ctr.c_reduce += cycles() - tr;
+    return len;
+  }
+  uint32_t out = 0, nbr = 0, ndrop = 0, nskip = 0;
+  uint64_t nb = ~0ull;
+  for (uint32_t base = 0; base < len; base += kWave) {
+    const uint32_t i = base + lane;
+    const bool valid = i < len;
+    uint32_t mi = kNone, rp = kNone;
+    uint64_t ks = 0, a[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) a[w] = 0;
+    if (valid) {
+      mi = recs[i].minidx & ~kSkip;
+      rp = recs[i].rep;
+      ks = recs[i].ksum;
+#pragma unroll
+      for (int w = 0; w < W; ++w) a[w] = recs[i].a[w];
+    }
+    const bool excl = valid && rp == kNone && mi < minrep;
+    const bool push = valid && rp != kNone;
+    bool drop = false, sk = push && mi < rp;
+    for (int d = 0; d < nd; ++d) {
+      const uint32_t pd = dpos[d];
+      bool inside = true, eq = true;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const uint64_t dw = m.dom[d * kMaxW + w];
+        inside = inside && (a[w] & ~dw) == 0;
+        eq = eq && a[w] == dw;
+      }
+      if (inside) {
+        drop = drop || (excl && i != pd && !(eq && i < pd));
+        sk = sk || push;
+      }
+    }
+    if (!P.dom_skip) sk = false;
+    const bool keep = valid && !drop;
+    const uint64_t km = ballot(keep);
+    const uint32_t pos = out + lanes_below(km);
+    wave_mem_wait();  // every lane has read its entry before any is overwritten
+    if (keep) {
+      recs[pos].minidx = mi | (sk ? kSkip : 0u);
+      recs[pos].rep = rp;
+      recs[pos].ksum = ks;
+#pragma unroll
+      for (int w = 0; w < W; ++w) recs[pos].a[w] = a[w];
+      if (push && !sk) {
+        const uint64_t c = ((uint64_t)rp << 32) | pos;
+        nb = c < nb ? c : nb;
+      }
+    }
+    nbr += __popcll(ballot(keep && push && !sk));
+    nskip += __popcll(ballot(keep && sk));
+    ndrop += __popcll(ballot(valid && drop));
+    out += __popcll(km);
+  }
+  wave_mem_wait();
+  best = (wave_min_pair(nb));
+  cbranch = nbr;
+  ctr.dropped += ndrop;
+  ctr.skipped += nskip;
+  if (P.profile) ctr.c_reduce += cycles() - tr;
+  return out;
 }
 
 // Whether sharing a node is worthwhile now: the top-level vertices are all
@@ -1090,7 +1332,7 @@ __device__ __forceinline__ uint32_t share_wanted(const Params& P, uint64_t gw, u
 template <int W, bool KL_R, bool C_R>
 __device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC, Frame F,
                     uint32_t& lds_top, uint64_t& hbm_top, Counters& ctr, const TaskArrays& ta,
-                    uint64_t root_rec, uint32_t root_len, uint32_t nR0, uint64_t gw, Frame* fr);
+                    uint64_t root_rec, uint32_t root_len, uint32_t nR0, uint64_t gw, uint32_t fb);
 
 // What to do with a child node (list at crec, committed in the arenas): run it
 // with re-indexed one-word bitsets when its L fits one word and the task's
@@ -1102,7 +1344,7 @@ __device__ bool child_node(const Params& P, const WaveMem& m, uint64_t gw, Count
                            uint32_t nL, uint32_t nC, const TaskArrays& ta, Rec<W>* crec,
                            uint32_t& clen, uint64_t& best_child, uint32_t& cbranch, uint32_t nb,
                            const uint64_t* ax, uint64_t sumR, uint32_t& lds_top,
-                           uint64_t& hbm_top, Frame* fr_next) {
+                           uint64_t& hbm_top, uint32_t fb_next) {
   const uint32_t lane = lane_id();
   if constexpr (W > 1) {
     if (nb <= 64 && !P.list) {
@@ -1143,6 +1385,12 @@ __device__ bool child_node(const Params& P, const WaveMem& m, uint64_t gw, Count
           if (P.profile) ctr.c_merge += cycles() - tm;
           ctr.merges++;
         }
+        dlen = reduce_list<1>(P, m, dst, dlen, dbest, dcb, ctr);
+        if (dcb == 0) {  // every push of the node is known to be non-maximal
+          lds_top = sl;
+          hbm_top = sh;
+          return false;
+        }
         TaskArrays tc{kLc, ta.C};
         bool shared = false;
         const uint32_t cx = P.share && (uint64_t)dlen * dcb >= (uint64_t)P.share_work
@@ -1168,7 +1416,7 @@ __device__ bool child_node(const Params& P, const WaveMem& m, uint64_t gw, Count
           G.nbits = nb;
           G.tier = dt;
           G.top = (uint32_t)dtop;
-          dfs<1, false, C_R>(P, m, nb, nC, G, lds_top, hbm_top, ctr, tc, 0, 0, 0, gw, fr_next);
+          dfs<1, false, C_R>(P, m, nb, nC, G, lds_top, hbm_top, ctr, tc, 0, 0, 0, gw, fb_next);
         }
         lds_top = sl;
         hbm_top = sh;
@@ -1185,6 +1433,8 @@ __device__ bool child_node(const Params& P, const WaveMem& m, uint64_t gw, Count
     if (P.profile) ctr.c_merge += cycles() - tm;
     ctr.merges++;
   }
+  clen = reduce_list<W>(P, m, crec, clen, best_child, cbranch, ctr);
+  if (cbranch == 0) return false;  // every push of the node is known to be non-maximal
   const uint32_t cx = P.share && !P.list && (uint64_t)clen * cbranch >= (uint64_t)P.share_work
                           ? share_wanted(P, gw, nL, nC, clen, W)
                           : kNone;
@@ -1210,7 +1460,7 @@ __device__ bool child_node(const Params& P, const WaveMem& m, uint64_t gw, Count
 template <int W, bool KL_R, bool C_R>
 __device__ bool late_share(const Params& P, const WaveMem& m, uint64_t gw, uint32_t nL,
                            uint32_t nC,
-                           const TaskArrays& ta, Frame& F, Frame* fr, uint32_t depth,
+                           const TaskArrays& ta, Frame& F, uint32_t fb, uint32_t depth,
                            Counters& ctr) {
   // cheap gate first: the top-level vertices are all claimed and the queue is
   // short; then pick the shallowest frame with pushes left
@@ -1220,14 +1470,10 @@ __device__ bool late_share(const Params& P, const WaveMem& m, uint64_t gw, uint3
   Frame S;
   const uint32_t scan = depth < 4u ? depth : 4u;
   for (uint32_t i = 0; i < scan; ++i) {
-    const uint32_t cur = uni(fr[i].cur);
-    if (cur != kNone) {
+    const Frame Fi = frame_get(m, fb + i);
+    if (Fi.cur != kNone) {
       at = i;
-      S.rec = uni64(fr[i].rec);
-      S.len = uni(fr[i].len);
-      S.curidx = uni(fr[i].curidx);
-      S.nbits = uni(fr[i].nbits);
-      S.sumR = uni64(fr[i].sumR);
+      S = Fi;
       break;
     }
   }
@@ -1250,8 +1496,8 @@ __device__ bool late_share(const Params& P, const WaveMem& m, uint64_t gw, uint3
     F.cur = kNone;
     return true;
   }
-  if (lane_id() == 0) fr[at].cur = kNone;
-  wave_mem_wait();
+  S.cur = kNone;
+  frame_put(m, fb + at, S);
   return false;
 }
 
@@ -1260,7 +1506,7 @@ __device__ bool late_share(const Params& P, const WaveMem& m, uint64_t gw, uint3
 template <int W, bool KL_R, bool C_R>
 __device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC, Frame F,
                     uint32_t& lds_top, uint64_t& hbm_top, Counters& ctr, const TaskArrays& ta,
-                    uint64_t root_rec, uint32_t root_len, uint32_t nR0, uint64_t gw, Frame* fr) {
+                    uint64_t root_rec, uint32_t root_len, uint32_t nR0, uint64_t gw, uint32_t fb) {
   typedef Rec<W> R_t;
   const uint32_t lane = lane_id();
   const bool prune = P.prune != 0;
@@ -1280,24 +1526,15 @@ __device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC,
       if (F.tier == 0) lds_top = F.top; else hbm_top = F.top;
       if (depth == 0) break;
       --depth;
-      wave_mem_wait();
-      F = fr[depth];
-      F.sumR = uni64(F.sumR);
-      F.rec = uni64(F.rec);
-      F.len = uni(F.len);
-      F.cur = uni(F.cur);
-      F.curidx = uni(F.curidx);
-      F.nbits = uni(F.nbits);
-      F.tier = uni(F.tier);
-      F.top = uni(F.top);
+      F = frame_get(m, fb + depth);
       continue;
     }
-    if (P.late_share && !P.list && (tick & 7u) == 0 &&
-        late_share<W, KL_R, C_R>(P, m, gw, nL, nC, ta, F, fr, depth, ctr))
+    if (P.late_share && !P.list && (tick & 31u) == 0 &&
+        late_share<W, KL_R, C_R>(P, m, gw, nL, nC, ta, F, fb, depth, ctr))
       continue;  // the current frame was handed over: F.cur is now kNone
     R_t* recs = (R_t*)F.rec;
     const uint32_t xpos = F.cur, xidx = F.curidx;
-    {
+    if (P.count_bytes) {
       // B(n) = 4 (deg(x) + |L_parent| + |C|)
       uint32_t xc = rd<C_R>(ta.C + xidx);
       uint64_t dx = (uint64_t)(voff[xc + 1] - voff[xc]);
@@ -1381,7 +1618,7 @@ __device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC,
               for (int w = 0; w < W; ++w) s = s && ((root[i].a[w] & ax[w]) == ax[w]);
             }
             uint64_t sm = ballot(s);
-            if (s) out[2 + nb + c + lanes_below(sm)] = m.C[root[i].minidx];
+            if (s) out[2 + nb + c + lanes_below(sm)] = m.C[root[i].minidx & ~kSkip];
             c += __popcll(sm);
           }
         }
@@ -1395,13 +1632,12 @@ __device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC,
       else hbm_top = ctop + (uint64_t)o.clen * sizeof(R_t);
       bool here = child_node<W, KL_R, C_R>(P, m, gw, ctr, nL, nC, ta, crec, o.clen, o.best_child,
                                            o.cbranch, nb, ax, sumR, lds_top, hbm_top,
-                                           fr + depth);
+                                           fb + depth);
       lds_top = sl;
       hbm_top = sh;
       if (!here) continue;
       // descend: the child list becomes a frame
-      if (lane == 0) fr[depth] = F;
-      wave_mem_wait();
+      frame_put(m, fb + depth, F);
       ++depth;
       if (ctier == 0) lds_top = (uint32_t)(ctop + (uint64_t)o.clen * sizeof(R_t));
       else hbm_top = ctop + (uint64_t)o.clen * sizeof(R_t);
@@ -1424,12 +1660,32 @@ __device__ void run_items_w(const Params& P, const WaveMem& m, CtxHdr* h, uint32
                             uint32_t word, uint32_t cnt, Counters& ctr, uint64_t gw) {
   typedef Rec<W> R_t;
   R_t* recs;
-  TaskArrays ta;
-  ctx_arrays<W>(h, nL, nC, len, recs, ta);
+  TaskArrays rta;
+  ctx_arrays<W>(h, nL, nC, len, recs, rta);
+  // the hash keys of L, read at every maximal push below: a local copy at
+  // the bottom of the arenas (LDS when it fits)
+  uint32_t lds0 = 0;
+  uint64_t hbm0 = 0;
+  uint32_t kt = 1;
+  uint64_t ktop = 0;
+  const uint64_t kbytes = ((uint64_t)nL * 8 + 15) & ~15ull;
+  uint64_t kaddr = arena_take(m, lds0, hbm0, kbytes, kt, ktop);
+  if (kaddr == 0) {
+    ctr.overflow += cnt;
+    ctr.ovf_arena += cnt;
+    if (lane_id() == 0)
+      __hip_atomic_fetch_add(&h->refs, (unsigned long long)(-(long long)cnt), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  if (kt == 0) lds0 = (uint32_t)kbytes; else hbm0 = kbytes;
+  for (uint32_t j = lane_id(); j < nL; j += kWave) ((uint64_t*)kaddr)[j] = ld_dev(rta.kL + j);
+  wave_mem_wait();
+  const TaskArrays ta{(const uint64_t*)kaddr, rta.C};
   for (uint32_t it = 0; it < cnt; ++it) {
     const uint32_t xpos = read_lane(word, 2 + 2 * it), xidx = read_lane(word, 3 + 2 * it);
     uint64_t t0 = P.profile ? cycles() : 0;
-    {
+    if (P.count_bytes) {
       uint32_t xc = ld_dev(ta.C + xidx);
       uint64_t dx = (uint64_t)(m.g->v_off[xc + 1] - m.g->v_off[xc]);
       ctr.balg += 4ull * (dx + nbits + nC);
@@ -1441,8 +1697,8 @@ __device__ void run_items_w(const Params& P, const WaveMem& m, CtxHdr* h, uint32
       ax[w] = uni64(ld_dev(&recs[xpos].a[w]));
       nb += __popcll(ax[w]);
     }
-    uint32_t lds_top = 0;
-    uint64_t hbm_top = 0;
+    uint32_t lds_top = lds0;
+    uint64_t hbm_top = hbm0;
     uint32_t ctier = 1;
     uint64_t ctop = 0;
     uint64_t crec_addr = arena_take(m, lds_top, hbm_top, (uint64_t)len * sizeof(R_t), ctier, ctop);
@@ -1465,15 +1721,15 @@ __device__ void run_items_w(const Params& P, const WaveMem& m, CtxHdr* h, uint32
       phase(ctr, P.profile, 5, t0);
       continue;
     }
-    const uint64_t sumL = key_sum_of_bits<W, true>(ax, ta);
+    const uint64_t sumL = key_sum_of_bits<W, false>(ax, ta);
     const uint64_t sumR = sumR0 + o.absorbed;
     add_biclique(ctr, biclique_hash(sumL, sumR, m.g->swapped));
     if (nb > 1 && o.best_child != ~0ull) {
       if (ctier == 0) lds_top = (uint32_t)(ctop + (uint64_t)o.clen * sizeof(R_t));
       else hbm_top = ctop + (uint64_t)o.clen * sizeof(R_t);
-      bool here = child_node<W, true, true>(P, m, gw, ctr, nL, nC, ta, (R_t*)crec_addr, o.clen,
-                                            o.best_child, o.cbranch, nb, ax, sumR, lds_top,
-                                            hbm_top, m.fr);
+      bool here = child_node<W, false, true>(P, m, gw, ctr, nL, nC, ta, (R_t*)crec_addr, o.clen,
+                                             o.best_child, o.cbranch, nb, ax, sumR, lds_top,
+                                             hbm_top, 0);
       if (here) {
         if (ctier == 0) lds_top = (uint32_t)(ctop + (uint64_t)o.clen * sizeof(R_t));
         else hbm_top = ctop + (uint64_t)o.clen * sizeof(R_t);
@@ -1486,7 +1742,7 @@ __device__ void run_items_w(const Params& P, const WaveMem& m, CtxHdr* h, uint32
         F.nbits = nb;
         F.tier = ctier;
         F.top = (uint32_t)ctop;
-        dfs<W, true, true>(P, m, nL, nC, F, lds_top, hbm_top, ctr, ta, 0, 0, 0, gw, m.fr);
+        dfs<W, false, true>(P, m, nL, nC, F, lds_top, hbm_top, ctr, ta, 0, 0, 0, gw, 0);
       }
     }
     phase(ctr, P.profile, 5, t0);
@@ -1559,12 +1815,30 @@ __device__ void enumerate_task(const Params& P, const WaveMem& m, const uint32_t
   // group twins (not in list mode: its R' lists need one entry per candidate)
   uint32_t len = nC;
   const uint64_t list_addr = braw;
+  uint64_t best = ~0ull;
+  uint32_t nbr = 0;
   if (P.group && !P.list && nC > 1) {
-    uint64_t bc = 0;
-    uint32_t cb = 0;
-    len = merge_twins<Wt>(m, brec, nC, lds_top, hbm_top, bc, cb);
+    len = merge_twins<Wt>(m, brec, nC, lds_top, hbm_top, best, nbr);
   }
+  if (len == nC) {
+    // first push: the smallest rep
+    for (uint32_t i = lane; i < len; i += kWave) {
+      const uint32_t rp = brec[i].rep;
+      if (rp != kNone) {
+        const uint64_t c = ((uint64_t)rp << 32) | i;
+        best = c < best ? c : best;
+      }
+    }
+    best = (wave_min_pair(best));
+  }
+  // drop dominated excluded candidates and mark the non-maximal pushes (not in
+  // list mode: its R' lists read the root list)
+  if (!P.list) len = reduce_list<Wt>(P, m, brec, len, best, nbr, ctr);
   wave_mem_wait();
+  if (best == ~0ull) {  // nothing to push below the task's own biclique
+    phase(ctr, P.profile, 5, t0);
+    return;
+  }
   TaskArrays lta{m.kL, m.C};
   if (share_ctx != kNone &&
       publish_list<Wt, false, false>(P, gw, share_ctx, graph_id(P, m), (const R_t*)list_addr, len, nL, nC, nL,
@@ -1573,34 +1847,33 @@ __device__ void enumerate_task(const Params& P, const WaveMem& m, const uint32_t
     phase(ctr, P.profile, 6, t0);
     return;
   }
+  // the hash keys of L, read at every maximal push: into LDS when small
+  const uint64_t* kL = m.kL;
+  uint32_t lds0 = 0;
+  const uint32_t kbytes = (nL * 8 + 15) & ~15u;
+  if (kbytes <= m.lds_bytes / 4) {
+    for (uint32_t j = lane; j < nL; j += kWave) ((uint64_t*)m.lds)[j] = m.kL[j];
+    kL = (const uint64_t*)m.lds;
+    lds0 = kbytes;
+  }
   // final root list: into LDS when it fits, else at the HBM arena's base
   uint64_t root = list_addr;
   uint32_t rtier = 1;
-  if ((uint64_t)len * sizeof(R_t) <= m.lds_bytes) {
-    R_t* dst = (R_t*)m.lds;
+  if ((uint64_t)lds0 + (uint64_t)len * sizeof(R_t) <= m.lds_bytes) {
+    R_t* dst = (R_t*)(m.lds + lds0);
     const R_t* src = (const R_t*)list_addr;
     for (uint32_t i = lane; i < len; i += kWave) dst[i] = src[i];
-    root = (uint64_t)m.lds;
+    root = (uint64_t)dst;
     rtier = 0;
-    lds_top = (uint32_t)((uint64_t)len * sizeof(R_t));
+    lds_top = lds0 + (uint32_t)((uint64_t)len * sizeof(R_t));
     hbm_top = 0;
   } else {
+    lds_top = lds0;
     hbm_top = (uint64_t)len * sizeof(R_t);
   }
   wave_mem_wait();
-  // first push: the smallest rep
-  uint64_t best = ~0ull;
-  const R_t* rl = (const R_t*)root;
-  for (uint32_t i = lane; i < len; i += kWave) {
-    uint32_t rp = rl[i].rep;
-    if (rp != kNone) {
-      uint64_t c = ((uint64_t)rp << 32) | i;
-      best = c < best ? c : best;
-    }
-  }
-  best = uni64(wave_min64(best));
   phase(ctr, P.profile, 4, t0);
-  if (best != ~0ull) {
+  {
     Frame F;
     F.sumR = sumR;
     F.rec = root;
@@ -1609,9 +1882,9 @@ __device__ void enumerate_task(const Params& P, const WaveMem& m, const uint32_t
     F.curidx = (uint32_t)(best >> 32);
     F.nbits = nL;
     F.tier = rtier;
-    F.top = 0;
-    TaskArrays ta{m.kL, m.C};
-    dfs<Wt, false, false>(P, m, nL, nC, F, lds_top, hbm_top, ctr, ta, root, len, nR, gw, m.fr);
+    F.top = rtier == 0 ? lds0 : 0;
+    TaskArrays ta{kL, m.C};
+    dfs<Wt, false, false>(P, m, nL, nC, F, lds_top, hbm_top, ctr, ta, root, len, nR, gw, 0);
   }
   phase(ctr, P.profile, 5, t0);
 }
@@ -1677,7 +1950,7 @@ __device__ void run_task(const Params& P, const WaveMem& m, const Task& t, int k
     L = m.L;
   }
   if (nL == 0) {  // isolated vertex: no biclique (undefined in the reference, Q5)
-    ctr.balg += 4ull * sdeg;
+    if (P.count_bytes) ctr.balg += 4ull * sdeg;
     ctr.nonmax++;
     return;
   }
@@ -1695,7 +1968,7 @@ __device__ void run_task(const Params& P, const WaveMem& m, const Task& t, int k
       if (o0 == 0) or_wg(bm1 + (w >> 12), 1ull << ((w >> 6) & 63));
     }
   });
-  ctr.balg += 4ull * (sdeg + root_edges);
+  if (P.count_bytes) ctr.balg += 4ull * (sdeg + root_edges);
   ctr.edges += root_edges;
   wave_mem_wait();
   phase(ctr, P.profile, 2, t0);
@@ -1821,7 +2094,11 @@ __device__ void run_task(const Params& P, const WaveMem& m, const Task& t, int k
     if (bytes <= P.w.ctx_bytes) share_ctx = free_ctx(P, gw);
   }
   const bool share = share_ctx != kNone;
-  if (!share && ((rule && k < P.max_prefix) || (must && k < kMaxPrefix))) {
+  // a large task with no context slot free: its children are recomputed by
+  // other waves (recompute, the reference's LargeToTiny) or, by default, the
+  // task runs here and late sharing hands its frames out when waves run dry
+  const bool recompute = rule && k < P.max_prefix && (P.recompute || !P.share || P.list);
+  if (!share && (recompute || (must && k < kMaxPrefix))) {
     // children S + [C[i]], i >= next_id, each recomputed by its wave
     const unsigned long long ne = (rest + kChunk - 1) / kChunk;
     if (lane == 0)
@@ -1852,7 +2129,7 @@ __device__ void run_task(const Params& P, const WaveMem& m, const Task& t, int k
     clear_cmap();
     ctr.tasks--;
     ctr.nodes--;
-    ctr.balg -= 4ull * (sdeg + root_edges);
+    if (P.count_bytes) ctr.balg -= 4ull * (sdeg + root_edges);
     unsigned long long at = ~0ull;
     if (lane == 0) {
       at = __hip_atomic_fetch_add(P.s.n_deferred, 1ull, __ATOMIC_RELAXED,

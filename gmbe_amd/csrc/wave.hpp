@@ -71,6 +71,53 @@ __device__ __forceinline__ uint32_t wave_incl_scan32(uint32_t x) {
 }
 __device__ __forceinline__ int first_lane(uint64_t m) { return __builtin_ctzll(m); }
 
+// Maximum over the wave by DPP: row_shr 1/2/4/8 leave each row's maximum in
+// its last lane, row_bcast:15 and row_bcast:31 carry rows into lane 63.
+// Lanes whose DPP source lies outside the row read 0 (the identity).
+__device__ __forceinline__ uint32_t dpp_max_step(uint32_t x, uint32_t y) { return x > y ? x : y; }
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
+  x = dpp_max_step(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false));
+  x = dpp_max_step(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false));
+  x = dpp_max_step(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false));
+  x = dpp_max_step(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false));
+  x = dpp_max_step(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false));
+  x = dpp_max_step(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false));
+  return __builtin_amdgcn_readlane(x, 63);
+}
+// Minimum over the wave (as the maximum of the complement).
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) { return ~wave_max_u32(~x); }
+
+// Minimum of (hi << 32 | lo) over the wave when no two lanes share the
+// smallest hi (or all hold ~0): a 32-bit minimum, then the winner's lo.
+__device__ __forceinline__ uint64_t wave_min_pair(uint64_t c) {
+  const uint32_t hi = (uint32_t)(c >> 32);
+  const uint32_t m = wave_min_u32(hi);
+  const uint64_t at = __builtin_amdgcn_ballot_w64(hi == m);
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)c, (uint32_t)__builtin_ctzll(at));
+  return ((uint64_t)m << 32) | lo;
+}
+
+// Sum over the wave by DPP, the same row_shr / row_bcast steps as the
+// maximum, with 64-bit adds of the shifted halves.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint64_t dpp_add64(uint64_t x) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)x, CTRL, ROWS, 0xf, false);
+  const uint32_t hi =
+      (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(x >> 32), CTRL, ROWS, 0xf, false);
+  return x + (((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
+  x = dpp_add64<0x111, 0xf>(x);
+  x = dpp_add64<0x112, 0xf>(x);
+  x = dpp_add64<0x114, 0xf>(x);
+  x = dpp_add64<0x118, 0xf>(x);
+  x = dpp_add64<0x142, 0xa>(x);
+  x = dpp_add64<0x143, 0xc>(x);
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)x, 63);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(x >> 32), 63);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // Ordering of this wave's own memory operations across its lanes.  Within a
 // work-group (one CU) the hardware keeps them coherent, so this is a
 // compiler-level fence (no cache maintenance is emitted).

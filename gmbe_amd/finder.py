@@ -60,7 +60,8 @@ class IterFinderGpu6:
     STAT_NAMES = ["cyc_wait", "cyc_prefix", "cyc_expand", "cyc_extract", "cyc_build", "cyc_dfs",
                   "cyc_split", "edges", "entries", "ovf_arena", "ovf_wide", "ovf_defer",
                   "entries_w1", "nodes_w1", "cyc_tail", "sub_pass", "sub_merge",
-                  "sub_compress", "sub_publish", "publishes", "merges", "abandoned", "late"]
+                  "sub_compress", "sub_publish", "publishes", "merges", "abandoned", "late",
+                  "sub_reduce", "dropped", "skipped"]
 
     def stats(self) -> dict:
         out = (C.c_uint64 * 32)()

@@ -214,11 +214,13 @@ def test_scheduler_knobs_keep_the_result(gpu, knobs):
 
 def test_unsplit_matches_reference_traversal(gpu):
     """Without splitting, the pushes, non-maximal verdicts and algorithmic
-    bytes are those of the reference's traversal (the oracle restates it)."""
+    bytes are those of the reference's traversal (the oracle restates it).
+    Dropping dominated excluded entries keeps every verdict (dom_k on); the
+    skip marks leave out pushes known to be non-maximal, so they are off."""
     g = workloads.run_graph(1)
     o = oracle.gmbe(g.oracle_graph())
-    for group in (0, 1):
-        r = gpu_digest(g, sel=0, group=group)
+    for group, dom_k in ((0, 0), (1, 0), (0, 8), (1, 8)):
+        r = gpu_digest(g, sel=0, group=group, dom_k=dom_k, dom_skip=0, count_bytes=1)
         assert r.digest() == o.digest()
         assert r.nodes == o.nodes
         assert r.non_maximal == o.non_maximal
