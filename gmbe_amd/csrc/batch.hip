@@ -24,7 +24,7 @@ constexpr int kClasses = 4;
 int size_class(uint64_t E) { return E < 2000 ? 0 : E < 8000 ? 1 : E < 32000 ? 2 : 3; }
 
 // kernel variants the autotuner tries: (waves per block, LDS KB per wave)
-constexpr int kVariants[][2] = {{4, 8}, {4, 0}, {4, 4}, {4, 16}, {8, 4}, {2, 16}};
+constexpr int kVariants[][2] = {{4, 7}, {4, 0}, {4, 3}, {4, 16}, {8, 7}, {2, 7}};
 // split-threshold candidates per class: (bound_height, bound_size)
 constexpr int kBounds[][2] = {{20, 1500}, {1 << 30, 1 << 30}, {10, 500}, {40, 6000}};
 
@@ -345,7 +345,7 @@ static int gmbe_batch_set_impl(gmbe_batch* b, const char* key, int64_t value) {
     if ((rc = range(2, 8))) return rc;
     // keep the LDS arena if that variant is built, else take the one that is
     int lds = kn.lds_kb;
-    if (!pick_kernel(lds, (int)value, false).fn) lds = value == 8 ? 4 : value == 2 ? 16 : 8;
+    if (!pick_kernel(lds, (int)value, false).fn) lds = 7;
     return set_variant(b, (int)value, lds);
   }
   if (k == "lds_kb") {

@@ -48,6 +48,14 @@ struct KernelChoice {
 // per wave in KB; `wide` selects the 16-word-bitset kernel of the second pass.
 // Returns {nullptr} for a combination that is not built.
 KernelChoice pick_kernel(int lds_kb, int waves, bool wide);
+// the variants (kern_*.hip)
+KernelChoice kernel_w4_l7();
+KernelChoice kernel_w4_l3();
+KernelChoice kernel_w4_l0();
+KernelChoice kernel_w4_l16();
+KernelChoice kernel_w8_l7();
+KernelChoice kernel_w2_l7();
+KernelChoice kernel_wide();
 // Resident blocks of a variant on this device (persistent grid size).
 int resident_blocks(const KernelChoice& kc, int cu_count);
 

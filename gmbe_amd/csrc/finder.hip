@@ -18,135 +18,6 @@ using namespace gmbe;
 
 namespace mbe {
 
-// Persistent grid: every wave loops over tasks until none are left.
-// OCC: waves per SIMD the register allocation must leave room for (also
-// imposed on the device functions the kernel calls).
-template <int WAVES, int LDS_PER_WAVE, int WMAX, int OCC>
-__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
-void mbe_persistent(Params P) {
-  const uint64_t t_start = P.exit_stamps ? wall_clock64() : 0;
-  if (P.budget != 0) P.deadline = wall_clock64() + P.budget;
-  __shared__ __attribute__((aligned(16))) char lds[WAVES * (LDS_PER_WAVE > 0 ? LDS_PER_WAVE : 16)];
-  __shared__ uint32_t s_off[WAVES][kWave + 1];
-  __shared__ int64_t s_start[WAVES][kWave];
-  __shared__ Frame s_fr[WAVES][kLdsFrames];
-  const uint32_t wv = threadIdx.x / kWave;
-  const uint64_t gw = (uint64_t)blockIdx.x * WAVES + wv;
-  WaveMem m = wave_mem(P, gw, lds + wv * (LDS_PER_WAVE > 0 ? LDS_PER_WAVE : 16), LDS_PER_WAVE,
-                       s_off[wv], s_start[wv], s_fr[wv]);
-  Counters c;
-  Task t;
-  int k = 0;
-  uint32_t word = 0;
-  for (;;) {
-    uint64_t tw = P.profile ? cycles() : 0;
-    int kind = next_work(P, word, t, k, (uint32_t)(gw & (kShards - 1)));
-    if (kind == 0) {
-      // the final wait (nothing left for this wave) goes to the split slot's
-      // neighbour so that mid-run waiting and the tail can be told apart
-      if (P.profile) c.tail += cycles() - tw;
-      break;
-    }
-    phase(c, P.profile, 0, tw);
-    uint32_t gid;
-    if (kind == 2) {
-      gid = t.gid;
-      m.g = P.gt + gid;
-      Prefix pre{nullptr, 0, 0};
-      if (k > 1) pre = prefix_common(*m.g, m, t, k - 1);
-      run_task<WMAX>(P, m, t, k, pre, c, gw);
-    } else if (read_lane(word, 0) & kItemKind) {
-      // items of a shared task context (its header names the graph)
-      gid = uni(ld_dev(&ctx_hdr(P, read_lane(word, 1))->gid));
-      m.g = P.gt + gid;
-      run_items<WMAX>(P, m, word, c, gw);
-    } else {
-      // an entry: children prefix + [c_i] of one split task; the prefix's
-      // common neighbourhood is computed once for all of them
-      const uint32_t hdr = read_lane(word, 0);
-      const int kp = (int)child_hdr_k(hdr);
-      const uint32_t cnt = child_hdr_cnt(hdr);
-      gid = child_hdr_gid(hdr);
-      m.g = P.gt + gid;
-      for (int i = 0; i < kMaxPrefix; ++i) t.v[i] = i < kp ? read_lane(word, 1 + i) : kNone;
-      t.gid = gid;
-      const Prefix pre = prefix_common(*m.g, m, t, kp);
-      for (uint32_t i = 0; i < cnt; ++i) {
-        t.v[kp] = read_lane(word, 8 + i);
-        run_task<WMAX>(P, m, t, kp + 1, pre, c, gw);
-      }
-    }
-    if (P.per_graph) flush_graph(P, c, gid);
-    work_done(P);
-    graph_work(P, gid, -1);
-  }
-  if (P.exit_stamps) {
-    // the reference's per-SM exit times (src/IterFinderGpu.cu:1038-1041): once
-    // every wave of the block is done, the block's start and exit stamps
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      P.exit_stamps[2 * blockIdx.x] = t_start;
-      P.exit_stamps[2 * blockIdx.x + 1] = wall_clock64();
-    }
-  }
-  if (lane_id() == 0) {
-    unsigned long long* a = P.acc;
-    __hip_atomic_fetch_add(a + A_MAXIMAL, (unsigned long long)c.maximal, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_add(a + A_NONMAX, (unsigned long long)c.nonmax, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_add(a + A_HSUM, (unsigned long long)c.hsum, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_xor(a + A_HXOR, (unsigned long long)c.hxor, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_add(a + A_NODES, (unsigned long long)c.nodes, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_add(a + A_TASKS, (unsigned long long)c.tasks, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_add(a + A_SPLITS, (unsigned long long)c.splits, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_add(a + A_OVERFLOW, (unsigned long long)c.overflow, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_add(a + A_BALG, (unsigned long long)c.balg, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    if (P.profile) {
-      for (int i = 0; i < 7; ++i)
-        __hip_atomic_fetch_add(a + A_CYC_WAIT + i, (unsigned long long)c.cyc[i], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(a + A_EDGES, (unsigned long long)c.edges, __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(a + A_ENTRIES, (unsigned long long)c.entries, __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(a + A_ENT_W1, (unsigned long long)c.ent_w1, __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(a + A_NODES_W1, (unsigned long long)c.nodes_w1, __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(a + A_CYC_TAIL, (unsigned long long)c.tail, __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-      const uint64_t extra[6] = {c.c_pass, c.c_merge, c.c_compress, c.c_publish, c.publishes,
-                                 c.merges};
-      for (int i = 0; i < 6; ++i)
-        __hip_atomic_fetch_add(a + A_CYC_PASS + i, (unsigned long long)extra[i],
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __hip_atomic_fetch_add(a + A_OVF_ARENA, (unsigned long long)c.ovf_arena, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_add(a + A_OVF_WIDE, (unsigned long long)c.ovf_wide, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_add(a + A_OVF_DEFER, (unsigned long long)c.ovf_defer, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_add(a + A_ABANDONED, (unsigned long long)c.abandoned, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_add(a + A_LATE, (unsigned long long)c.late, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    const uint64_t red[3] = {c.c_reduce, c.dropped, c.skipped};
-    for (int i = 0; i < 3; ++i)
-      if (red[i])
-        __hip_atomic_fetch_add(a + A_CYC_REDUCE + i, (unsigned long long)red[i], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
-
 // Mark every shared-context slot free (a run cut short by its deadline can
 // leave items of a context unrun).
 __global__ void reset_ctx(Params P, unsigned long long n) {
@@ -218,21 +89,18 @@ using mbe::kCtlStride;
 using mbe::kShards;
 
 KernelChoice pick_kernel(int lds_kb, int waves, bool wide) {
-  using namespace mbe;
-  // every kernel shares the stack-machine functions, whose registers are
-  // allocated for the laxest caller: all of them ask for two waves per SIMD
-  if (wide) return {mbe_persistent<4, 8192, kMaxW, 2>, 4, 8192};
-  if (waves == 8 && lds_kb == 4) return {mbe_persistent<8, 4096, kMainW, 2>, 8, 4096};
-  if (waves == 2 && lds_kb == 16) return {mbe_persistent<2, 16384, kMainW, 2>, 2, 16384};
+  // four waves per SIMD where the LDS allows (a 7 KB list arena per wave
+  // leaves room for four blocks of four waves per CU); the wide-bitset
+  // kernel needs two.  Each variant is compiled in its own kern_*.hip.
+  if (wide) return kernel_wide();
+  if (waves == 8 && lds_kb == 7) return kernel_w8_l7();
+  if (waves == 2 && lds_kb == 7) return kernel_w2_l7();
   if (waves != 4) return {nullptr, 0, 0};
   switch (lds_kb) {
-    case 7: return {mbe_persistent<4, 7168, kMainW, 4>, 4, 7168};
-    case 0: return {mbe_persistent<4, 0, kMainW, 2>, 4, 0};
-    case 4: return {mbe_persistent<4, 4096, kMainW, 2>, 4, 4096};
-    case 8: return {mbe_persistent<4, 8192, kMainW, 2>, 4, 8192};
-    case 12: return {mbe_persistent<4, 12288, kMainW, 2>, 4, 12288};
-    case 16: return {mbe_persistent<4, 16384, kMainW, 2>, 4, 16384};
-    case 24: return {mbe_persistent<4, 24576, kMainW, 2>, 4, 24576};
+    case 7: return kernel_w4_l7();
+    case 0: return kernel_w4_l0();
+    case 3: return kernel_w4_l3();
+    case 16: return kernel_w4_l16();
     default: return {nullptr, 0, 0};
   }
 }
@@ -405,7 +273,7 @@ struct gmbe_finder {
   const gmbe_graph* g = nullptr;
   int sel = 2, bound_height = 20, bound_size = 1500;
   // knobs
-  int64_t waves_per_block = 4, blocks = 0, prune = 1, max_prefix = 4, lds_kb = 16;
+  int64_t waves_per_block = 4, blocks = 0, prune = 1, max_prefix = 4, lds_kb = 7;
   int64_t queue_log2 = 22, list_cap = 0, arena_kb = 16384, root_stride = 1, root_phase = 0;
   int64_t root_begin = 0, root_end = -1;  // processing-order rank range, -1 = all
   int64_t deadline_ms = 0;                 // 0 = run to completion
@@ -450,7 +318,7 @@ struct gmbe_finder {
 using namespace gmbe;
 
 static int finder_alloc(gmbe_finder* f) {
-  KernelChoice kc = pick_kernel((int)f->lds_kb, 4, false);
+  KernelChoice kc = pick_kernel((int)f->lds_kb, (int)f->waves_per_block, false);
   int blocks = f->blocks > 0 ? (int)f->blocks : resident_blocks(kc, f->eng.cu_count);
   int nwaves = blocks * kc.waves;
   // unless set, the list arena and the context slots are sized for the graph:
@@ -562,13 +430,17 @@ static int gmbe_finder_set_impl(gmbe_finder* f, const char* key, int64_t value) 
     return GMBE_OK;
   };
   int rc;
-  if (k == "waves_per_block") { if ((rc = range(4, 4))) return rc; f->waves_per_block = value; }
+  if (k == "waves_per_block") {
+    if (value != 2 && value != 4 && value != 8)
+      return set_error(GMBE_ERR_ARG, "waves_per_block must be 2, 4 or 8");
+    f->waves_per_block = value;
+  }
   else if (k == "blocks") { if ((rc = range(0, 1 << 20))) return rc; f->blocks = value; }
   else if (k == "prune") { if ((rc = range(0, 1))) return rc; f->prune = value; }
   else if (k == "max_prefix") { if ((rc = range(2, mbe::kMaxPrefix))) return rc; f->max_prefix = value; }
   else if (k == "lds_kb") {
-    if (value != 0 && value != 4 && value != 7 && value != 8 && value != 12 && value != 16 && value != 24)
-      return set_error(GMBE_ERR_ARG, "lds_kb must be 0, 4, 7, 8, 12, 16 or 24");
+    if (value != 0 && value != 3 && value != 7 && value != 16)
+      return set_error(GMBE_ERR_ARG, "lds_kb must be 0, 3, 7 or 16");
     f->lds_kb = value;
   }
   else if (k == "queue_log2") { if ((rc = range(4, 30))) return rc; f->queue_log2 = value; }
@@ -681,7 +553,7 @@ static int gmbe_finder_execute_impl(gmbe_finder* f, gmbe_result* res) {
   P.count_bytes = (int)f->count_bytes;
   P.share_work = (int)std::min<int64_t>(f->share_work, INT32_MAX);
   P.share_queue = f->share_queue >= 0 ? (int)f->share_queue : f->nwaves;
-  KernelChoice kc = pick_kernel((int)f->lds_kb, 4, false);
+  KernelChoice kc = pick_kernel((int)f->lds_kb, (int)f->waves_per_block, false);
   hipLaunchKernelGGL(mbe::reset_ctx, dim3(64), dim3(256), 0, f->stream, P,
                      (unsigned long long)f->nwaves * (unsigned long long)f->ctx_slots);
   HIP_TRY(hipGetLastError());

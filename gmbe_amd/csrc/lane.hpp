@@ -30,7 +30,7 @@ struct LaneCaps {
 
 // Kernel knobs of a lane (shared with the finder's meaning).
 struct LaneKnobs {
-  int lds_kb = 8, waves = 4, blocks = 0;
+  int lds_kb = 7, waves = 4, blocks = 0;
   int64_t arena_kb = 2048, ctx_kb = 1024, ctx_slots = 4, queue_log2 = 20;
   int prune = 1, max_prefix = 4, group = 1, share = 1, split = 1;
   int64_t share_work = 4096, share_queue = -1, merge_min = 256, merge_branch = 16,

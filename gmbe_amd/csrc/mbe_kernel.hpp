@@ -144,23 +144,52 @@ struct Params {
   int recompute;             // a large task with no free context slot: split into recomputed children
   int count_bytes;           // count the algorithmic bytes of every node (alg_bytes)
   uint64_t budget;           // wall-clock ticks a wave may run (0 = no deadline)
-  uint64_t deadline;         // set per wave: its start + budget
   unsigned long long* exit_stamps;  // if set: wall clock at which each block's waves were done
 };
 
-struct Counters {
-  uint64_t maximal = 0, nonmax = 0, hsum = 0, hxor = 0, nodes = 0, tasks = 0, splits = 0,
-           overflow = 0;
-  uint64_t balg = 0;  // algorithmic bytes: 4 B per id of every list a node must read
-  uint64_t cyc[7] = {0, 0, 0, 0, 0, 0, 0};  // wait, prefix, expand, extract, build, dfs, split
-  uint64_t edges = 0, entries = 0, ovf_arena = 0, ovf_wide = 0, ovf_defer = 0;
-  uint64_t ent_w1 = 0, nodes_w1 = 0;  // the part of entries / pushes with one-word bitsets
-  uint64_t tail = 0;                   // cycles of the final wait for work
-  uint64_t c_pass = 0, c_merge = 0, c_compress = 0, c_publish = 0, publishes = 0, merges = 0;
-  uint64_t abandoned = 0;              // subtrees left unfinished at the deadline
-  uint64_t late = 0;                   // frames handed over by late sharing
-  uint64_t c_reduce = 0, dropped = 0, skipped = 0;  // list reduction: cycles, entries
+// A wave's counter, kept in LDS (the Counters of a wave live in its block's
+// shared memory): the first active lane adds with an LDS atomic whose result
+// is unused, so an update never waits, and no register holds the counter.
+struct LdsCount {
+  unsigned long long v;
+  __device__ __forceinline__ bool first() const { return lanes_below(ballot(true)) == 0; }
+  __device__ __forceinline__ void operator+=(uint64_t x) {
+    if (first())
+      __hip_atomic_fetch_add(as_lds(&v), (unsigned long long)x, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  __device__ __forceinline__ void operator-=(uint64_t x) { *this += (uint64_t)(0 - x); }
+  __device__ __forceinline__ void operator^=(uint64_t x) {
+    if (first())
+      __hip_atomic_fetch_xor(as_lds(&v), (unsigned long long)x, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  __device__ __forceinline__ void operator++(int) { *this += 1; }
+  __device__ __forceinline__ void operator--(int) { *this += (uint64_t)0 - 1; }
+  __device__ __forceinline__ uint64_t get() const {
+    return uni64(__hip_atomic_load(as_lds(&v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+  }
+  __device__ __forceinline__ void set(uint64_t x) {
+    if (first())
+      __hip_atomic_store(as_lds(&v), (unsigned long long)x, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
 };
+
+// Per-wave counters (one Counters per wave, in LDS; zeroed at kernel start).
+struct Counters {
+  LdsCount maximal, nonmax, hsum, hxor, nodes, tasks, splits, overflow;
+  LdsCount balg;  // algorithmic bytes: 4 B per id of every list a node must read
+  LdsCount cyc[7];  // wait, prefix, expand, extract, build, dfs, split
+  LdsCount edges, entries, ovf_arena, ovf_wide, ovf_defer;
+  LdsCount ent_w1, nodes_w1;  // the part of entries / pushes with one-word bitsets
+  LdsCount tail;              // cycles of the final wait for work
+  LdsCount c_pass, c_merge, c_compress, c_publish, publishes, merges;
+  LdsCount abandoned;         // subtrees left unfinished at the deadline
+  LdsCount late;              // frames handed over by late sharing
+  LdsCount c_reduce, dropped, skipped;  // list reduction: cycles, entries
+};
+constexpr int kCounters = (int)(sizeof(Counters) / sizeof(LdsCount));
 
 // Per-graph result counters of a batch (u64 each, kPerGraph per graph).
 enum PerGraph : int {
@@ -178,25 +207,30 @@ __device__ __forceinline__ void graph_work(const Params& P, uint32_t gid, long l
 
 // Move this wave's result counters into graph gid's slots (batch only).
 __device__ __forceinline__ void flush_graph(const Params& P, Counters& c, uint32_t gid) {
+  wave_mem_wait();
+  const uint64_t add[kPerGraph] = {c.maximal.get(), c.hsum.get(), 0, c.nonmax.get(),
+                                   c.nodes.get(), c.overflow.get(), c.balg.get(), c.tasks.get()};
+  const uint64_t hx = c.hxor.get();
   if (lane_id() == 0) {
     unsigned long long* a = P.per_graph + (uint64_t)gid * kPerGraph;
-    const uint64_t add[kPerGraph] = {c.maximal, c.hsum, 0, c.nonmax, c.nodes, c.overflow, c.balg,
-                                     c.tasks};
     for (int i = 0; i < kPerGraph; ++i)
       if (i != G_HXOR && add[i] != 0)
         __hip_atomic_fetch_add(a + i, (unsigned long long)add[i], __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
-    if (c.hxor != 0)
-      __hip_atomic_fetch_xor(a + G_HXOR, (unsigned long long)c.hxor, __ATOMIC_RELAXED,
+    if (hx != 0)
+      __hip_atomic_fetch_xor(a + G_HXOR, (unsigned long long)hx, __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_AGENT);
   }
-  c.maximal = c.hsum = c.hxor = c.nonmax = c.nodes = c.overflow = c.balg = c.tasks = 0;
+  LdsCount* z[] = {&c.maximal, &c.hsum, &c.hxor, &c.nonmax, &c.nodes, &c.overflow, &c.balg,
+                   &c.tasks};
+  for (LdsCount* q : z) q->set(0);
+  wave_mem_wait();
 }
 
 // True once this wave's deadline has passed (never without a budget).  The
 // constant-rate wall clock is read per wave; no memory is touched.
-__device__ __forceinline__ bool past_deadline(const Params& P) {
-  return P.budget != 0 && wall_clock64() > P.deadline;
+__device__ __forceinline__ bool past_deadline(const Params& P, uint64_t deadline) {
+  return P.budget != 0 && wall_clock64() > deadline;
 }
 
 __device__ __forceinline__ uint64_t cycles() { return __builtin_readcyclecounter(); }
@@ -241,7 +275,7 @@ __device__ __forceinline__ int64_t bsearch_u32(const uint32_t* a, int64_t n, uin
 
 // out = A ∩ B (ascending); the smaller list is lane-strided and each lane
 // binary-searches the larger one (seq_intersect_warp, src/GpuUtil.cuh:135-162).
-__device__ uint32_t wave_intersect(const uint32_t* A, uint32_t nA, const uint32_t* B,
+__device__ __forceinline__ uint32_t wave_intersect(const uint32_t* A, uint32_t nA, const uint32_t* B,
                                    uint32_t nB, uint32_t* out) {
   if (nA > nB) {
     const uint32_t* t = A; A = B; B = t;
@@ -264,7 +298,7 @@ __device__ uint32_t wave_intersect(const uint32_t* A, uint32_t nA, const uint32_
   return cnt;
 }
 
-__device__ uint32_t wave_intersect_count(const uint32_t* A, uint32_t nA, const uint32_t* B,
+__device__ __forceinline__ uint32_t wave_intersect_count(const uint32_t* A, uint32_t nA, const uint32_t* B,
                                          uint32_t nB) {
   if (nA > nB) {
     const uint32_t* t = A; A = B; B = t;
@@ -283,7 +317,7 @@ __device__ uint32_t wave_intersect_count(const uint32_t* A, uint32_t nA, const u
 // edges of 64 L vertices at a time are flattened across lanes; each lane
 // finds its vertex by binary search over the 64 prefix offsets kept in LDS.
 template <class F>
-__device__ uint64_t for_each_edge(const uint32_t* L, uint32_t nL, const int64_t* off,
+__device__ __forceinline__ uint64_t for_each_edge(const uint32_t* L, uint32_t nL, const int64_t* off,
                                   const uint32_t* adj, uint32_t* s_off, int64_t* s_start, F&& f) {
   uint32_t lane = lane_id();
   uint64_t edges = 0;
@@ -347,7 +381,7 @@ __device__ __forceinline__ uint64_t slot_of(const Sched& s, uint32_t sh, unsigne
 
 // Pop one entry into `word` (lane l < 32 holds word l), scanning the shards
 // from `home`.
-__device__ bool queue_pop(const Sched& s, uint32_t& word, uint32_t home) {
+__device__ __forceinline__ bool queue_pop(const Sched& s, uint32_t& word, uint32_t home) {
   const uint32_t lane = lane_id();
   if ((long long)uni64(ld_dev(s.queued)) <= 0) return false;
   unsigned long long got = ~0ull;
@@ -387,7 +421,7 @@ __device__ bool queue_pop(const Sched& s, uint32_t& word, uint32_t home) {
 
 // Reserve ne consecutive positions in one shard, trying `home` first.
 // Returns false when every shard lacks room.
-__device__ bool queue_reserve(const Sched& s, uint32_t home, uint32_t ne, uint32_t& sh_out,
+__device__ __forceinline__ bool queue_reserve(const Sched& s, uint32_t home, uint32_t ne, uint32_t& sh_out,
                               unsigned long long& start) {
   const uint32_t lane = lane_id();
   unsigned long long st = ~0ull;
@@ -416,7 +450,7 @@ __device__ bool queue_reserve(const Sched& s, uint32_t home, uint32_t ne, uint32
 
 // Push the children prefix+[cand[i]], i < n, as ceil(n / kChunk) entries.
 // All or nothing: false (nothing pushed) when the queue lacks room.
-__device__ bool queue_push_children(const Sched& s, const Task& prefix, int k,
+__device__ __forceinline__ bool queue_push_children(const Sched& s, const Task& prefix, int k,
                                     const uint32_t* cand, uint32_t n, uint32_t& n_entries,
                                     uint32_t home) {
   uint32_t lane = lane_id();
@@ -463,13 +497,14 @@ __device__ __forceinline__ unsigned long long roots_total(const Sched& s) {
 // every unit has finished (the gc/lc/tc protocol of
 // src/IterFinderGpu.cu:978-986,1020-1034 collapsed into one outstanding-work
 // counter), 1 for an entry (in `word`), 2 for a root task (in `t`, `k`).
-__device__ int next_work(const Params& P, uint32_t& word, Task& t, int& k, uint32_t home) {
+__device__ __forceinline__ int next_work(const Params& P, uint32_t& word, Task& t, int& k, uint32_t home,
+                         uint64_t deadline) {
   const Sched& s = P.s;
   uint32_t lane = lane_id();
   uint32_t nap = 1;
   const unsigned long long n_roots = roots_total(s);
   for (;;) {
-    if (past_deadline(P)) return 0;  // leave what is queued: the run is cut short
+    if (past_deadline(P, deadline)) return 0;  // leave what is queued: the run is cut short
     if (queue_pop(s, word, home)) return 1;
     unsigned long long r = ~0ull;
     if (lane == 0 && ld_dev(s.root_next) < n_roots) {
@@ -544,18 +579,48 @@ struct WaveMem {
   uint32_t* s_off; // 64 + 1 u32 in LDS
   int64_t* s_start;// 64 i64 in LDS
   const DevGraph* g;  // graph of the current unit of work (set per unit)
+  uint64_t deadline;  // wall clock past which this wave stops (with a budget)
 };
+
+// Whether p lies in this wave's LDS list arena.
+__device__ __forceinline__ bool in_lds(const WaveMem& m, const void* p) {
+  return (uint64_t)((const char*)p - m.lds) < (uint64_t)m.lds_bytes;
+}
 
 // Frame i of the wave's DFS stack: LDS for the shallow ones.
 __device__ __forceinline__ void frame_put(const WaveMem& m, uint32_t i, const Frame& F) {
   if (lane_id() == 0) {
-    if (i < kLdsFrames) m.fr_lds[i] = F;
-    else m.fr[i] = F;
+    if (i < kLdsFrames) {
+      auto* q = as_lds(m.fr_lds) + i;
+      q->sumR = F.sumR;
+      q->rec = F.rec;
+      q->len = F.len;
+      q->cur = F.cur;
+      q->curidx = F.curidx;
+      q->nbits = F.nbits;
+      q->tier = F.tier;
+      q->top = F.top;
+    } else {
+      m.fr[i] = F;
+    }
   }
   wave_mem_wait();
 }
 __device__ __forceinline__ Frame frame_get(const WaveMem& m, uint32_t i) {
-  Frame F = i < kLdsFrames ? m.fr_lds[i] : m.fr[i];
+  Frame F;
+  if (i < kLdsFrames) {
+    const auto* q = as_lds(m.fr_lds) + i;
+    F.sumR = q->sumR;
+    F.rec = q->rec;
+    F.len = q->len;
+    F.cur = q->cur;
+    F.curidx = q->curidx;
+    F.nbits = q->nbits;
+    F.tier = q->tier;
+    F.top = q->top;
+  } else {
+    F = m.fr[i];
+  }
   F.sumR = uni64(F.sumR);
   F.rec = uni64(F.rec);
   F.len = uni(F.len);
@@ -613,7 +678,7 @@ __device__ __forceinline__ void add_biclique(Counters& c, uint64_t h) {
 }
 
 // List mode: reserve and write a record [nL, nR, L..., R...] (internal ids).
-__device__ uint64_t list_reserve(const Params& P, uint64_t words) {
+__device__ __forceinline__ uint64_t list_reserve(const Params& P, uint64_t words) {
   unsigned long long at = ~0ull;
   if (lane_id() == 0) {
     at = __hip_atomic_fetch_add(P.list_cursor, (unsigned long long)words, __ATOMIC_RELAXED,
@@ -706,12 +771,15 @@ struct PassOut {
 // One push: x (candidate index xidx, bitset ax) against the parent's list
 // `recs`, writing the child list to `crec`.  LIST_REMOTE: the parent list is
 // another wave's shared context (read-only: no prune writes).
-template <int W, bool LIST_REMOTE>
-__device__ PassOut push_pass(Rec<W>* recs, uint32_t len, const uint64_t* ax, uint32_t xidx,
-                             Rec<W>* crec, bool prune) {
-  // U chunks of 64 entries per iteration (their loads are issued before the
-  // first is used); reduced lists are mostly a single chunk, so one
-  constexpr int U = 1;
+// ordered: the list is in ascending minidx and every pushable entry has
+// rep == minidx (reduce_list and merge_twins keep this when dom_skip is on),
+// so pushes go in position order and the next push of the parent, and the
+// first of the child, are the first qualifying positions (a ballot per
+// chunk); otherwise they are minimum reps over the wave.
+// LDS_IN / LDS_OUT: the parent / child list is in LDS (ds_ accesses).
+template <int W, bool LIST_REMOTE, bool LDS_IN = false, bool LDS_OUT = false>
+__device__ __forceinline__ PassOut push_pass(Rec<W>* recs, uint32_t len, const uint64_t* ax, uint32_t xidx,
+                             Rec<W>* crec, bool prune, bool ordered) {
   const uint32_t lane = lane_id();
   bool nonmax = false;
   // once the node is known to be non-maximal its child list is never used:
@@ -719,105 +787,135 @@ __device__ PassOut push_pass(Rec<W>* recs, uint32_t len, const uint64_t* ax, uin
   bool dead = false;
   uint64_t absorbed = 0, best_next = ~0ull, best_child = ~0ull;
   uint32_t clen = 0, cbranch = 0;
-  for (uint32_t base = 0; base < len; base += kWave * U) {
-    uint32_t mi[U], rp[U];
-    uint64_t ae[U][W], ks[U];
+  for (uint32_t base = 0; base < len; base += kWave) {
+    const uint32_t k = base + lane;
+    const bool valid = k < len;
+    uint32_t mi = kNone, rep = kNone;
+    uint64_t ks = 0, ae[W];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t k = base + u * kWave + lane;
-      mi[u] = kNone;
-      rp[u] = kNone;
-      ks[u] = 0;
+    for (int w = 0; w < W; ++w) ae[w] = 0;
+    if (valid) {
+      if constexpr (LDS_IN) {
+        const auto* q = as_lds(recs) + k;
+        mi = q->minidx;
+        rep = q->rep;
+        ks = q->ksum;
 #pragma unroll
-      for (int w = 0; w < W; ++w) ae[u][w] = 0;
-      if (k < len) {
-        mi[u] = rd<LIST_REMOTE>(&recs[k].minidx);
-        rp[u] = rd<LIST_REMOTE>(&recs[k].rep);
-        ks[u] = rd<LIST_REMOTE>(&recs[k].ksum);
+        for (int w = 0; w < W; ++w) ae[w] = q->a[w];
+      } else {
+        mi = rd<LIST_REMOTE>(&recs[k].minidx);
+        rep = rd<LIST_REMOTE>(&recs[k].rep);
+        ks = rd<LIST_REMOTE>(&recs[k].ksum);
 #pragma unroll
-        for (int w = 0; w < W; ++w) ae[u][w] = rd<LIST_REMOTE>(&recs[k].a[w]);
+        for (int w = 0; w < W; ++w) ae[w] = rd<LIST_REMOTE>(&recs[k].a[w]);
       }
     }
-    // maximality first, for the whole group: a group that shows the node is
+    // maximality first, for the whole chunk: a chunk that shows the node is
     // non-maximal writes none of its child entries
-    bool sub[U];
+    bool sub = valid, zero = true, inside = valid;
+    uint64_t me[W];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t k = base + u * kWave + lane;
-      sub[u] = k < len;
-#pragma unroll
-      for (int w = 0; w < W; ++w) sub[u] = sub[u] && ((ae[u][w] & ax[w]) == ax[w]);
-      if (sub[u]) {
-        if ((mi[u] & ~kSkip) < xidx) nonmax = true;  // a twin or superset before x
-        else absorbed += ks[u];                      // joins R'
-      }
+    for (int w = 0; w < W; ++w) {
+      me[w] = ae[w] & ax[w];
+      sub = sub && (me[w] == ax[w]);
+      zero = zero && (me[w] == 0);
+      inside = inside && ((ae[w] & ~ax[w]) == 0);
+    }
+    const uint32_t mim = mi & ~kSkip;
+    nonmax = nonmax || (sub && mim < xidx);  // a twin or superset before x
+    // groups that join R': few lanes, summed one by one
+    uint64_t am = ballot(sub && mim >= xidx);
+    while (am) {
+      const uint32_t l = (uint32_t)__builtin_ctzll(am);
+      am &= am - 1;
+      absorbed += read_lane64(ks, l);
     }
     dead = dead || ballot(nonmax) != 0;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t k = base + u * kWave + lane;
-      const bool valid = k < len;
-      const uint32_t rep = rp[u];
-      uint64_t me[W];
-      bool zero = true, inside = valid;
-#pragma unroll
-      for (int w = 0; w < W; ++w) {
-        me[w] = ae[u][w] & ax[w];
-        zero = zero && (me[w] == 0);
-        inside = inside && ((ae[u][w] & ~ax[w]) == 0);
+    const bool br = rep != kNone && rep > xidx;
+    const bool pruned = prune && valid && br && inside;
+    if constexpr (!LIST_REMOTE) {
+      if (pruned) {
+        if constexpr (LDS_IN) as_lds(recs)[k].rep = kNone;
+        else recs[k].rep = kNone;
       }
-      const bool br = rep != kNone && rep > xidx;
-      const bool pruned = prune && valid && br && inside;
-      if constexpr (!LIST_REMOTE) {
-        if (pruned) recs[k].rep = kNone;
-      }
-      const bool keep = !dead && valid && !sub[u] && !zero;
-      const uint32_t crep = br ? rep : kNone;  // branchability before this push's prune
-      const uint64_t km = ballot(keep);
-      const uint32_t pos = clen + lanes_below(km);
-      if (keep) {
-        crec[pos].minidx = mi[u] & ~kSkip;  // a skip mark holds at its own node only
+    }
+    const bool keep = !dead && valid && !sub && !zero;
+    const uint32_t crep = br ? rep : kNone;  // branchability before this push's prune
+    const uint64_t km = ballot(keep);
+    const uint32_t pos = clen + lanes_below(km);
+    if (keep) {
+      if constexpr (LDS_OUT) {
+        auto* q = as_lds(crec) + pos;
+        q->minidx = mim;  // a skip mark holds at its own node only
+        q->rep = crep;
+        q->ksum = ks;
+#pragma unroll
+        for (int w = 0; w < W; ++w) q->a[w] = me[w];
+      } else {
+        crec[pos].minidx = mim;
         crec[pos].rep = crep;
-        crec[pos].ksum = ks[u];
+        crec[pos].ksum = ks;
 #pragma unroll
         for (int w = 0; w < W; ++w) crec[pos].a[w] = me[w];
-        if (crep != kNone) {
-          uint64_t c = ((uint64_t)crep << 32) | pos;
-          best_child = c < best_child ? c : best_child;
+      }
+    }
+    const bool nx = valid && br && !pruned && !(mi & kSkip);
+    const bool cc = keep && crep != kNone;
+    if (ordered) {
+      if (best_next == ~0ull) {
+        const uint64_t bm = ballot(nx);
+        if (bm) {
+          const uint32_t l = (uint32_t)__builtin_ctzll(bm);
+          best_next = ((uint64_t)read_lane(rep, l) << 32) | (base + l);
         }
       }
-      if (valid && br && !pruned && !(mi[u] & kSkip)) {
-        uint64_t c = ((uint64_t)rep << 32) | k;
+      if (best_child == ~0ull) {
+        const uint64_t bm = ballot(cc);
+        if (bm) {
+          const uint32_t l = (uint32_t)__builtin_ctzll(bm);
+          best_child = ((uint64_t)read_lane(crep, l) << 32) | read_lane(pos, l);
+        }
+      }
+    } else {
+      if (nx) {
+        const uint64_t c = ((uint64_t)rep << 32) | k;
         best_next = c < best_next ? c : best_next;
       }
-      clen += __popcll(km);
-      cbranch += __popcll(ballot(keep && crep != kNone));
+      if (cc) {
+        const uint64_t c = ((uint64_t)crep << 32) | pos;
+        best_child = c < best_child ? c : best_child;
+      }
     }
+    clen += __popcll(km);
+    cbranch += __popcll(ballot(cc));
   }
   PassOut o;
   o.cbranch = cbranch;
-  o.nonmax = ballot(nonmax) != 0;
-  o.absorbed = wave_sum_u64(absorbed);
-  o.best_next = (wave_min_pair(best_next));
-  o.best_child = (wave_min_pair(best_child));
+  o.nonmax = dead;
+  o.absorbed = absorbed;
+  o.best_next = ordered ? best_next : wave_min_pair(best_next);
+  o.best_child = ordered ? best_child : wave_min_pair(best_child);
   o.clen = clen;
   return o;
 }
 
 template <int W, bool KL_R>
-__device__ __forceinline__ uint64_t key_sum_of_bits(const uint64_t* ax, const TaskArrays& ta) {
+__device__ __forceinline__ uint64_t key_sum_of_bits(const WaveMem& m, const uint64_t* ax,
+                                                    const TaskArrays& ta) {
   const uint32_t lane = lane_id();
   uint64_t sl = 0;
+  const bool lds = !KL_R && in_lds(m, ta.kL);
 #pragma unroll
   for (int w = 0; w < W; ++w)
-    if ((ax[w] >> lane) & 1ull) sl += rd<KL_R>(ta.kL + w * 64 + lane);
+    if ((ax[w] >> lane) & 1ull)
+      sl += lds ? as_lds(ta.kL)[w * 64 + lane] : rd<KL_R>(ta.kL + w * 64 + lane);
   return wave_sum_u64(sl);
 }
 
 // Re-index the bitsets of a list onto the set bits of ax (at most 64): entry
 // bit t of the output is the input's bit at the t-th set position of ax.
 template <int W>
-__device__ void compress_list(const Rec<W>* src, uint32_t len, const uint64_t* ax, Rec<1>* dst) {
+__device__ __forceinline__ void compress_list(const Rec<W>* src, uint32_t len, const uint64_t* ax, Rec<1>* dst) {
   const uint32_t lane = lane_id();
   for (uint32_t i = lane; i < len; i += kWave) {
     uint64_t out = 0;
@@ -904,7 +1002,7 @@ constexpr int kItemsPerEntry = 15;  // (position, candidate) pairs after hdr, ow
 // (nothing published) when the queue lacks room or fewer than min_items
 // entries are left to push.
 template <int W, bool KL_R, bool C_R>
-__device__ bool publish_list(const Params& P, uint64_t gw, uint32_t ctx, uint32_t gid,
+__device__ __forceinline__ bool publish_list(const Params& P, uint64_t gw, uint32_t ctx, uint32_t gid,
                              const Rec<W>* root, uint32_t len,
                              uint32_t nL, uint32_t nC, uint32_t nbits, uint64_t sumR0,
                              const TaskArrays& src, uint32_t min_rep = 0,
@@ -1019,8 +1117,9 @@ __device__ bool publish_list(const Params& P, uint64_t gw, uint32_t ctx, uint32_
 // pushable count; leaves the list unchanged (returns len) when the arenas
 // lack room for the table.
 template <int W>
-__device__ uint32_t merge_twins(const WaveMem& m, Rec<W>* crec, uint32_t len, uint32_t& lds_top,
-                                uint64_t& hbm_top, uint64_t& best_child, uint32_t& cbranch) {
+__device__ __forceinline__ uint32_t merge_twins(const WaveMem& m, Rec<W>* crec, uint32_t len, uint32_t& lds_top,
+                                uint64_t& hbm_top, uint64_t& best_child, uint32_t& cbranch,
+                                bool ordered) {
   const uint32_t lane = lane_id();
   struct Slot {
     unsigned long long id;  // hash << 32 | (first-inserted entry + 1); 0 = empty
@@ -1101,7 +1200,13 @@ __device__ uint32_t merge_twins(const WaveMem& m, Rec<W>* crec, uint32_t len, ui
     wave_mem_wait();  // every lane has read its entry before any is overwritten
     if (keep) {
       rp = ld_wg(&tab[h].rep);
-      crec[pos].minidx = ld_wg(&tab[h].mi);
+      const uint32_t mi = ld_wg(&tab[h].mi);
+      // ordered lists: a group whose smallest member is not its pushable one
+      // is non-maximal whenever pushed (that member contains it and comes
+      // first), here and below; it becomes excluded, so that every pushable
+      // entry keeps rep == minidx
+      if (ordered && rp != kNone && rp != mi) rp = kNone;
+      crec[pos].minidx = mi;
       crec[pos].rep = rp;
       crec[pos].ksum = ld_wg(&tab[h].ksum);
 #pragma unroll
@@ -1143,8 +1248,8 @@ __device__ uint32_t merge_twins(const WaveMem& m, Rec<W>* crec, uint32_t len, ui
 // is read from its lane, and the dominators are chosen and applied round by
 // round (a dropped entry is never chosen, and a chosen one is never dropped
 // later: a later dominator has no more bits and, if equal, a later lane).
-template <int W>
-__device__ uint32_t reduce_chunk(const Params& P, Rec<W>* recs, uint32_t len, int K,
+template <int W, bool LDS = false>
+__device__ __forceinline__ uint32_t reduce_chunk(const Params& P, Rec<W>* recs, uint32_t len, int K,
                                  uint32_t minrep, uint64_t& best, uint32_t& cbranch,
                                  Counters& ctr) {
   const uint32_t lane = lane_id();
@@ -1154,11 +1259,20 @@ __device__ uint32_t reduce_chunk(const Params& P, Rec<W>* recs, uint32_t len, in
 #pragma unroll
   for (int w = 0; w < W; ++w) a[w] = 0;
   if (valid) {
-    mi = recs[lane].minidx & ~kSkip;
-    rp = recs[lane].rep;
-    ks = recs[lane].ksum;
+    if constexpr (LDS) {
+      const auto* q = as_lds(recs) + lane;
+      mi = q->minidx & ~kSkip;
+      rp = q->rep;
+      ks = q->ksum;
 #pragma unroll
-    for (int w = 0; w < W; ++w) a[w] = recs[lane].a[w];
+      for (int w = 0; w < W; ++w) a[w] = q->a[w];
+    } else {
+      mi = recs[lane].minidx & ~kSkip;
+      rp = recs[lane].rep;
+      ks = recs[lane].ksum;
+#pragma unroll
+      for (int w = 0; w < W; ++w) a[w] = recs[lane].a[w];
+    }
   }
   const bool excl = valid && rp == kNone && mi < minrep;
   const bool push = valid && rp != kNone;
@@ -1194,30 +1308,49 @@ __device__ uint32_t reduce_chunk(const Params& P, Rec<W>* recs, uint32_t len, in
   wave_mem_wait();  // every lane has read its entry before any is overwritten
   uint64_t c = ~0ull;
   if (keep) {
-    recs[pos].minidx = mi | (sk ? kSkip : 0u);
-    recs[pos].rep = rp;
-    recs[pos].ksum = ks;
+    if constexpr (LDS) {
+      auto* q = as_lds(recs) + pos;
+      q->minidx = mi | (sk ? kSkip : 0u);
+      q->rep = rp;
+      q->ksum = ks;
 #pragma unroll
-    for (int w = 0; w < W; ++w) recs[pos].a[w] = a[w];
+      for (int w = 0; w < W; ++w) q->a[w] = a[w];
+    } else {
+      recs[pos].minidx = mi | (sk ? kSkip : 0u);
+      recs[pos].rep = rp;
+      recs[pos].ksum = ks;
+#pragma unroll
+      for (int w = 0; w < W; ++w) recs[pos].a[w] = a[w];
+    }
     if (push && !sk) c = ((uint64_t)rp << 32) | pos;
   }
   wave_mem_wait();
-  best = wave_min_pair(c);
-  cbranch = __popcll(ballot(keep && push && !sk));
-  ctr.dropped += __popcll(ballot(valid && drop));
-  ctr.skipped += __popcll(ballot(keep && sk));
+  const uint64_t bm = ballot(keep && push && !sk);
+  if (P.dom_skip) {  // ordered list: the first push is the first such position
+    best = bm ? read_lane64(c, (uint32_t)__builtin_ctzll(bm)) : ~0ull;
+  } else {
+    best = wave_min_pair(c);
+  }
+  cbranch = __popcll(bm);
+  if (P.profile) {
+    ctr.dropped += __popcll(ballot(valid && drop));
+    ctr.skipped += __popcll(ballot(keep && sk));
+  }
   return __popcll(km);
 }
 
 template <int W>
-__device__ uint32_t reduce_list(const Params& P, const WaveMem& m, Rec<W>* recs, uint32_t len,
+__device__ __forceinline__ uint32_t reduce_list(const Params& P, const WaveMem& m, Rec<W>* recs, uint32_t len,
                                 uint64_t& best, uint32_t& cbranch, Counters& ctr) {
   const uint32_t lane = lane_id();
   const int K = P.dom_k < kMaxDom ? P.dom_k : kMaxDom;
   if (K <= 0 || best == ~0ull || len < 2 || len >= (1u << 20)) return len;
   const uint64_t tr = P.profile ? cycles() : 0;
   if (len <= (uint32_t)kWave) {
-    const uint32_t n = reduce_chunk<W>(P, recs, len, K, (uint32_t)(best >> 32), best, cbranch, ctr);
+    const uint32_t n =
+        in_lds(m, recs)
+            ? reduce_chunk<W, true>(P, recs, len, K, (uint32_t)(best >> 32), best, cbranch, ctr)
+            : reduce_chunk<W, false>(P, recs, len, K, (uint32_t)(best >> 32), best, cbranch, ctr);
     if (P.profile) ctr.c_reduce += cycles() - tr;
     return n;
   }
@@ -1330,7 +1463,7 @@ __device__ __forceinline__ uint32_t share_wanted(const Params& P, uint64_t gw, u
 }
 
 template <int W, bool KL_R, bool C_R>
-__device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC, Frame F,
+__device__ __forceinline__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC, Frame F,
                     uint32_t& lds_top, uint64_t& hbm_top, Counters& ctr, const TaskArrays& ta,
                     uint64_t root_rec, uint32_t root_len, uint32_t nR0, uint64_t gw, uint32_t fb);
 
@@ -1340,7 +1473,7 @@ __device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC,
 // is scarce.  Returns true when the caller should descend into (crec, clen)
 // itself; false when the node has been handled here.
 template <int W, bool KL_R, bool C_R>
-__device__ bool child_node(const Params& P, const WaveMem& m, uint64_t gw, Counters& ctr,
+__device__ __forceinline__ bool child_node(const Params& P, const WaveMem& m, uint64_t gw, Counters& ctr,
                            uint32_t nL, uint32_t nC, const TaskArrays& ta, Rec<W>* crec,
                            uint32_t& clen, uint64_t& best_child, uint32_t& cbranch, uint32_t nb,
                            const uint64_t* ax, uint64_t sumR, uint32_t& lds_top,
@@ -1381,7 +1514,7 @@ __device__ bool child_node(const Params& P, const WaveMem& m, uint64_t gw, Count
         uint64_t dbest = best_child;
         if (P.group && dlen >= (uint32_t)P.merge_min && dcb >= (uint32_t)P.merge_branch) {
           uint64_t tm = P.profile ? cycles() : 0;
-          dlen = merge_twins<1>(m, dst, dlen, lds_top, hbm_top, dbest, dcb);
+          dlen = merge_twins<1>(m, dst, dlen, lds_top, hbm_top, dbest, dcb, P.dom_skip != 0);
           if (P.profile) ctr.c_merge += cycles() - tm;
           ctr.merges++;
         }
@@ -1429,7 +1562,7 @@ __device__ bool child_node(const Params& P, const WaveMem& m, uint64_t gw, Count
   if (P.group && !P.list && clen >= (uint32_t)P.merge_min &&
       cbranch >= (uint32_t)P.merge_branch) {
     uint64_t tm = P.profile ? cycles() : 0;
-    clen = merge_twins<W>(m, crec, clen, lds_top, hbm_top, best_child, cbranch);
+    clen = merge_twins<W>(m, crec, clen, lds_top, hbm_top, best_child, cbranch, P.dom_skip != 0);
     if (P.profile) ctr.c_merge += cycles() - tm;
     ctr.merges++;
   }
@@ -1458,7 +1591,7 @@ __device__ bool child_node(const Params& P, const WaveMem& m, uint64_t gw, Count
 // unpushable in place), so publishing that suffix and closing the frame keeps
 // the traversal's result.  Returns true when the current frame was closed.
 template <int W, bool KL_R, bool C_R>
-__device__ bool late_share(const Params& P, const WaveMem& m, uint64_t gw, uint32_t nL,
+__device__ __forceinline__ bool late_share(const Params& P, const WaveMem& m, uint64_t gw, uint32_t nL,
                            uint32_t nC,
                            const TaskArrays& ta, Frame& F, uint32_t fb, uint32_t depth,
                            Counters& ctr) {
@@ -1504,7 +1637,7 @@ __device__ bool late_share(const Params& P, const WaveMem& m, uint64_t gw, uint3
 // The stack machine from frame F (a list in this wave's arenas); saved
 // frames go to fr[0..).
 template <int W, bool KL_R, bool C_R>
-__device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC, Frame F,
+__device__ __forceinline__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC, Frame F,
                     uint32_t& lds_top, uint64_t& hbm_top, Counters& ctr, const TaskArrays& ta,
                     uint64_t root_rec, uint32_t root_len, uint32_t nR0, uint64_t gw, uint32_t fb) {
   typedef Rec<W> R_t;
@@ -1516,7 +1649,7 @@ __device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC,
   for (;;) {
     ++tick;
     // one wall-clock read per push: a push over a long list costs far more
-    if (P.budget != 0 && past_deadline(P)) {
+    if (P.budget != 0 && past_deadline(P, m.deadline)) {
       ctr.abandoned++;
       graph_work(P, graph_id(P, m), 1);  // never finished: the graph stays incomplete
       return;
@@ -1544,7 +1677,7 @@ __device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC,
     uint32_t nb = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-      ax[w] = uni64(recs[xpos].a[w]);
+      ax[w] = uni64(F.tier == 0 ? as_lds(recs)[xpos].a[w] : recs[xpos].a[w]);
       nb += __popcll(ax[w]);
     }
     // speculative child list (at most as long as the parent's)
@@ -1560,7 +1693,20 @@ __device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC,
     }
     R_t* crec = (R_t*)crec_addr;
     uint64_t tp = P.profile ? cycles() : 0;
-    PassOut o = push_pass<W, false>(recs, F.len, ax, xidx, crec, prune);
+    const bool ord = P.dom_skip != 0;
+    PassOut o;
+    if constexpr (W == 1) {
+      if (F.tier == 0 && ctier == 0)
+        o = push_pass<1, false, true, true>(recs, F.len, ax, xidx, crec, prune, ord);
+      else if (F.tier == 0)
+        o = push_pass<1, false, true, false>(recs, F.len, ax, xidx, crec, prune, ord);
+      else if (ctier == 0)
+        o = push_pass<1, false, false, true>(recs, F.len, ax, xidx, crec, prune, ord);
+      else
+        o = push_pass<1, false, false, false>(recs, F.len, ax, xidx, crec, prune, ord);
+    } else {
+      o = push_pass<W, false>(recs, F.len, ax, xidx, crec, prune, ord);
+    }
     if (P.profile) ctr.c_pass += cycles() - tp;
     ctr.nodes++;
     ctr.entries += F.len;
@@ -1574,7 +1720,7 @@ __device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC,
       ctr.nonmax++;
       continue;
     }
-    const uint64_t sumL = key_sum_of_bits<W, KL_R>(ax, ta);
+    const uint64_t sumL = key_sum_of_bits<W, KL_R>(m, ax, ta);
     const uint64_t sumR = F.sumR + o.absorbed;
     add_biclique(ctr, biclique_hash(sumL, sumR, swapped));
     if constexpr (!KL_R && !C_R) {
@@ -1655,7 +1801,7 @@ __device__ void dfs(const Params& P, const WaveMem& m, uint32_t nL, uint32_t nC,
 
 // Run the items of one entry (`word`: lane l holds word l).
 template <int W>
-__device__ void run_items_w(const Params& P, const WaveMem& m, CtxHdr* h, uint32_t nL,
+__device__ __forceinline__ void run_items_w(const Params& P, const WaveMem& m, CtxHdr* h, uint32_t nL,
                             uint32_t nC, uint32_t len, uint32_t nbits, uint64_t sumR0,
                             uint32_t word, uint32_t cnt, Counters& ctr, uint64_t gw) {
   typedef Rec<W> R_t;
@@ -1708,7 +1854,7 @@ __device__ void run_items_w(const Params& P, const WaveMem& m, CtxHdr* h, uint32
       continue;
     }
     uint64_t tp = P.profile ? cycles() : 0;
-    PassOut o = push_pass<W, true>(recs, len, ax, xidx, (R_t*)crec_addr, false);
+    PassOut o = push_pass<W, true>(recs, len, ax, xidx, (R_t*)crec_addr, false, P.dom_skip != 0);
     if (P.profile) ctr.c_pass += cycles() - tp;
     ctr.nodes++;
     ctr.entries += len;
@@ -1721,7 +1867,7 @@ __device__ void run_items_w(const Params& P, const WaveMem& m, CtxHdr* h, uint32
       phase(ctr, P.profile, 5, t0);
       continue;
     }
-    const uint64_t sumL = key_sum_of_bits<W, false>(ax, ta);
+    const uint64_t sumL = key_sum_of_bits<W, false>(m, ax, ta);
     const uint64_t sumR = sumR0 + o.absorbed;
     add_biclique(ctr, biclique_hash(sumL, sumR, m.g->swapped));
     if (nb > 1 && o.best_child != ~0ull) {
@@ -1753,7 +1899,7 @@ __device__ void run_items_w(const Params& P, const WaveMem& m, CtxHdr* h, uint32
 }
 
 template <int WMAX>
-__device__ void run_items(const Params& P, const WaveMem& m, uint32_t word, Counters& ctr,
+__device__ __forceinline__ void run_items(const Params& P, const WaveMem& m, uint32_t word, Counters& ctr,
                           uint64_t gw) {
   const uint32_t cnt = read_lane(word, 0) & ~kItemKind;
   const uint32_t owner = read_lane(word, 1);
@@ -1777,7 +1923,7 @@ __device__ void run_items(const Params& P, const WaveMem& m, uint32_t word, Coun
 // then grouped by bitset through a per-wave hash table), then either publish
 // it as a shared context (share) or run the stack machine here.
 template <int Wt>
-__device__ void enumerate_task(const Params& P, const WaveMem& m, const uint32_t* L, uint32_t nL,
+__device__ __forceinline__ void enumerate_task(const Params& P, const WaveMem& m, const uint32_t* L, uint32_t nL,
                                uint32_t nC, uint32_t next_id, uint64_t sumR, uint32_t nR,
                                Counters& ctr, uint64_t& t0, uint32_t share_ctx, uint64_t gw) {
   typedef Rec<Wt> R_t;
@@ -1818,7 +1964,7 @@ __device__ void enumerate_task(const Params& P, const WaveMem& m, const uint32_t
   uint64_t best = ~0ull;
   uint32_t nbr = 0;
   if (P.group && !P.list && nC > 1) {
-    len = merge_twins<Wt>(m, brec, nC, lds_top, hbm_top, best, nbr);
+    len = merge_twins<Wt>(m, brec, nC, lds_top, hbm_top, best, nbr, P.dom_skip != 0);
   }
   if (len == nC) {
     // first push: the smallest rep
@@ -1900,7 +2046,7 @@ struct Prefix {
   uint64_t sdeg;  // sum of the prefix vertices' degrees
 };
 
-__device__ Prefix prefix_common(const DevGraph& g, const WaveMem& m, const Task& t, int kp) {
+__device__ __forceinline__ Prefix prefix_common(const DevGraph& g, const WaveMem& m, const Task& t, int kp) {
   const uint32_t lane = lane_id();
   Prefix p;
   uint32_t v0 = t.v[0];
@@ -1927,7 +2073,7 @@ __device__ Prefix prefix_common(const DevGraph& g, const WaveMem& m, const Task&
 // One task S = t.v[0..k).  For k > 1, `pre` is the common neighbourhood of
 // S[0..k-1) (prefix_common).
 template <int WMAX>
-__device__ void run_task(const Params& P, const WaveMem& m, const Task& t, int k,
+__device__ __forceinline__ void run_task(const Params& P, const WaveMem& m, const Task& t, int k,
                          const Prefix& pre, Counters& ctr, uint64_t gw) {
   const DevGraph g = *m.g;
   const uint32_t lane = lane_id();

@@ -118,6 +118,17 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// A generic pointer known to address LDS, as an LDS pointer: accesses through
+// it are ds_ instructions instead of flat ones.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(3))) T* as_lds(T* p) {
+  return (__attribute__((address_space(3))) T*)p;
+}
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(3))) T* as_lds(const T* p) {
+  return (const __attribute__((address_space(3))) T*)p;
+}
+
 // Ordering of this wave's own memory operations across its lanes.  Within a
 // work-group (one CU) the hardware keeps them coherent, so this is a
 // compiler-level fence (no cache maintenance is emitted).

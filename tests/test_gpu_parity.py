@@ -85,7 +85,7 @@ def test_config1_schedulers(gpu, sel, bh, bs, mp):
     assert r.digest() == base
 
 
-@pytest.mark.parametrize("lds_kb", [0, 4, 8, 16, 24])
+@pytest.mark.parametrize("lds_kb", [0, 3, 7, 16])
 def test_lds_tiers(gpu, lds_kb):
     g = workloads.run_graph(2, 0.01)
     base = oracle.gmbe(g.oracle_graph(), threads=8).digest()

@@ -101,7 +101,7 @@ def test_finder_knob_errors_are_codes():
         return
     try:
         for key, val in [(b"deadline_ms", -1), (b"root_end", -2), (b"ctx_slots", 0),
-                         (b"lds_kb", 7), (b"nonsense", 1)]:
+                         (b"lds_kb", 5), (b"waves_per_block", 3), (b"nonsense", 1)]:
             assert _lib.lib().gmbe_finder_set(out, key, val) == -1
     finally:
         _lib.lib().gmbe_finder_destroy(out)
