@@ -1,26 +1,31 @@
 """Benchmark: maximal biclique enumeration on one MI355X.
 
-Workload (a step): the power-law bipartite graph at full size (Chung-Lu
-gamma=2.1 on both sides, |U|=100,000, |V|=200,000, 1.5M distinct edges, degree
-cap 20,000, seed 2; loaded as -t 1 -o 1 -f; bound_height=20, bound_size=1500),
-enumerated through IterFinderGpu6::Execute over the top-level vertices of
-processing rank >= 1024, i.e. the 98,976 lightest of the 100,000 task roots
-(about 1.9e8 maximal bicliques).  The 1,024 heaviest roots hold the graph's
-hub core: they yield more than 4e8 maximal bicliques in the first 40 s of each
-of their rank ranges and do not fit a benchmark step; they are measured
-beside this one, in rank ranges with the finder's deadline knob.
+Workload: the power-law bipartite graph at full size (Chung-Lu gamma=2.1 on
+both sides, |U|=100,000, |V|=200,000, 1.5M distinct edges, degree cap 20,000,
+seed 2; loaded as -t 1 -o 1 -f; bound_height=20, bound_size=1500), enumerated
+whole through IterFinderGpu6::Execute.  The K timed steps partition its
+100,000 top-level vertices: step i runs the vertices of processing rank
+r = i (mod K).  So the timed region is exactly one enumeration of the whole
+graph, and the line's count and hash are those of the whole graph (checked
+against the recorded value in profiles/ when one is there).
 
-The graph is resident in HBM before the timed region.  The JSON line reports
-maximal bicliques per second over the timed steps, the roofline of the
-enumeration kernel (algorithmic bytes per launch / kernel time from HIP events
-on the finder's stream, against the 8 TB/s HBM peak) and the CPU baseline:
-the restatement of the reference's CPU finder (MbeaFinder with MbeaAdvFinder's
-2-hop top level) timed on this host's cores over a bounded sample of the same
-top-level vertices.  Beside it, cpu_baseline_imbea and cpu_baseline_pmbe time
-the restatements of the authors' faster CPU finders (ImbeaAdvFinder and
-PmbeAdvFinder of baselines/MBE.zip) on samples of the same graph.
+The graph is resident in HBM before the timed region.  Untimed warmup: the
+first warmup step counts algorithmic bytes (the per-node byte model) on a
+stride-97 sample of all ranks; further ones run a light sample.  The JSON
+line reports maximal bicliques per second over the timed steps, the roofline
+of the enumeration kernel (algorithmic bytes per launch, i.e. the sampled
+bytes per node times the nodes of the launch, over the kernel's average
+launch time from HIP events on the finder's stream; the measured HBM traffic
+per launch from the committed counter summary; the traffic bandwidth beside
+it) and the CPU baseline: the restatement of the reference's CPU finder
+(MbeaFinder with MbeaAdvFinder's 2-hop top level) on all of this host's
+hardware threads over a bounded root sample, with the GPU timed on the same
+sample.  cpu_baseline_imbea and cpu_baseline_pmbe time the restatements of
+the authors' faster CPU finders (ImbeaAdvFinder, PmbeAdvFinder of
+baselines/MBE.zip) on the same sample.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py --only 1             # time only the first of the K parts
     python bench.py --workload batch     # 4,096 small graphs: batched vs looped launches
     python bench.py --workload serve     # Poisson requests at 2,000/s: p50/p99 latency
     python bench.py --copy-calibration   # plain 1 GiB device copy, for HBM counters
@@ -39,56 +44,69 @@ sys.path.insert(0, ROOT)
 METRIC = ("MBE wall time + maximal bicliques/s on 1 MI355X; p50/p99 latency of served "
           "queries")
 HBM_PEAK = 8.0e12  # bytes/s, MI355X
-ROOT_BEGIN = 1024  # first processing rank of the benchmark step
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "mbe_persistent_config2_hbm_traffic.json")
+DIGEST_FILE = os.path.join(ROOT, "profiles", "config2_full_digest.json")
+WORKLOAD = ("MBE Chung-Lu gamma=2.1 |U|=100,000 |V|=200,000 1.5M edges cap 20,000 seed 2; "
+            "-t 1 -o 1 -f; bound_height=20 bound_size=1500; whole graph, all 100,000 "
+            "top-level vertices, split into K launches by processing rank mod K")
+# CPU baseline sample: light top-level vertices the CPU finishes in seconds
+CPU_STRIDE, CPU_FIRST = 499, 4096
 
 
-def workload_name(config: int, scale: float, root_begin: int) -> str:
-    base = ("Chung-Lu gamma=2.1 |U|=100,000 |V|=200,000 1.5M edges cap 20,000 seed 2"
-            if config == 2 else f"config {config} graph of gmbe_amd/workloads.py")
-    if scale != 1.0:
-        base += f", sizes x{scale:g}"
-    return (f"MBE {base}; -t 1 -o 1 -f; bound_height=20 bound_size=1500; "
-            f"top-level vertices of processing rank >= {root_begin}")
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
-def cpu_baseline(graph, root_begin: int, seconds: float, algo: str = "mbea"):
-    """The oracle's MbeaFinder restatement on a sample of the step's roots;
-    algo="imbea" or "pmbe" runs the authors' ImbeaAdvFinder or PmbeAdvFinder
-    restatement instead.  PMBE numbers V by its containment order, so its
-    ranks select other vertices than the step's; the order is built once per
-    call, before the timed search (its time is reported apart)."""
+def cpu_baseline(graph, seconds: float, algo: str = "mbea"):
+    """The oracle's MbeaFinder restatement over the ranks CPU_FIRST,
+    CPU_FIRST + CPU_STRIDE, ... (algo="imbea" or "pmbe": the authors'
+    ImbeaAdvFinder or PmbeAdvFinder restatement; PMBE numbers V by its
+    containment order, so its ranks select other vertices, and its order is
+    built before the timed search) on every hardware thread of this host,
+    cut at `seconds`."""
     import oracle
     run, name = {
         "mbea": (oracle.mbea, "MbeaFinder restatement (oracle/oracle.c orc_mbea"),
         "imbea": (oracle.imbea, "ImbeaAdvFinder restatement (oracle/oracle.c orc_imbea"),
         "pmbe": (oracle.pmbe, "PmbeAdvFinder restatement (oracle/oracle.c orc_pmbe"),
     }[algo]
-    threads = min(16, os.cpu_count() or 1)
-    stride = 97
+    threads = os.cpu_count() or 1
     og = graph.oracle_graph()
-    maximal, secs, wall, tasks, phases = 0, 0.0, 0.0, 0, 0
-    for j in range(1 if algo == "pmbe" else stride):
-        t0 = time.perf_counter()
-        r = run(og, threads=threads, stride=stride, phase=root_begin + j,
-                time_limit=max(1.0, seconds - secs))
-        wall += time.perf_counter() - t0
-        maximal += r.maximal
-        secs += r.seconds
-        tasks += r.tasks
-        phases += 1
-        if secs >= seconds:
-            break
-    order = (f"; ranks in the containment order, built in {wall - secs:.1f} s before the search"
-             if algo == "pmbe" else "")
+    n_roots = max(0, (graph.stats()["V_size"] - CPU_FIRST + CPU_STRIDE - 1) // CPU_STRIDE)
+    t0 = time.perf_counter()
+    r = run(og, threads=threads, stride=CPU_STRIDE, phase=CPU_FIRST, time_limit=seconds)
+    wall = time.perf_counter() - t0
+    complete = r.tasks >= n_roots
+    order = (f"; ranks in the containment order, built in {wall - r.seconds:.1f} s before the "
+             "search" if algo == "pmbe" else "")
     return {
-        "value": maximal / secs if secs > 0 else 0.0, "unit": "bicliques/s", "cores": threads,
-        "kind": "port",
-        "sample": (f"{name}, {threads} threads) over "
-                   f"{tasks} top-level vertices of the same {'graph' if order else 'step'} "
-                   f"(ranks {root_begin}+j+97t, "
-                   f"j < {phases}{order}): {maximal} maximal bicliques in {secs:.1f} s"),
+        "value": r.maximal / r.seconds if r.seconds > 0 else 0.0, "unit": "bicliques/s",
+        "cores": threads, "kind": "port", "cpu": cpu_model(),
+        "sample": (f"{name}, {threads} threads) over top-level vertices of processing rank "
+                   f"{CPU_FIRST} + {CPU_STRIDE} t ({n_roots} vertices{order}): {r.tasks} done, "
+                   f"{r.maximal} maximal bicliques in {r.seconds:.1f} s"
+                   + ("" if complete else " (cut by the time limit)")),
+        "complete": bool(complete), "digest": [int(r.maximal), int(r.hash_sum), int(r.hash_xor)],
     }
+
+
+def gpu_on_cpu_sample(graph):
+    """The GPU on the CPU baseline's root sample (same vertices, same digest)."""
+    from gmbe_amd import IterFinderGpu6, workloads
+    f = IterFinderGpu6(graph, workloads.BOUND_HEIGHT, workloads.BOUND_SIZE,
+                       root_begin=CPU_FIRST, root_stride=CPU_STRIDE,
+                       root_phase=CPU_FIRST % CPU_STRIDE)
+    f.execute()
+    r = f.execute()
+    f.close()
+    return r
 
 
 def copy_calibration():
@@ -104,7 +122,7 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
 
 
-def measured_traffic(workload: str):
+def measured_traffic():
     """HBM bytes per launch of the enumeration kernel from the committed
     counter summary, when it was taken on this workload."""
     try:
@@ -112,7 +130,7 @@ def measured_traffic(workload: str):
             t = json.load(fh)
     except (OSError, ValueError):
         return None
-    if t.get("workload") != workload:
+    if t.get("workload") != WORKLOAD:
         return None
     return t.get("hbm_bytes_per_launch")
 
@@ -348,11 +366,9 @@ def bench_serve(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=2)
-    ap.add_argument("--scale", type=float, default=1.0)
-    ap.add_argument("--root-begin", type=int, default=ROOT_BEGIN)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--only", type=int, default=0, help="time only the first N of the K parts")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
@@ -385,69 +401,103 @@ def main():
     from gmbe_amd import IterFinderGpu6, workloads
 
     t0 = time.time()
-    graph = workloads.run_graph(args.config, args.scale)
+    graph = workloads.run_graph(2)
     st = graph.stats()
     print(f"# graph: |U|={st['U_size']} |V|={st['V_size']} |E|={st['edge_size']} "
           f"(built in {time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
-    f = IterFinderGpu6(graph, workloads.BOUND_HEIGHT, workloads.BOUND_SIZE,
-                       root_begin=args.root_begin)
+    K = max(1, args.steps)
+    f = IterFinderGpu6(graph, workloads.BOUND_HEIGHT, workloads.BOUND_SIZE)
+    # warmup: byte model on a stride-97 sample of all ranks, then light samples
+    bytes_per_node = None
     for i in range(args.warmup):
+        f.set("root_stride", 97)
+        f.set("root_phase", 0 if i == 0 else 1024 % 97)
+        f.set("root_begin", 0 if i == 0 else 1024)
+        f.set("count_bytes", 1 if i == 0 else 0)
         r = f.execute()
+        if i == 0:
+            bytes_per_node = r.alg_bytes / max(1, r.nodes)
         print(f"# warmup {i}: {r.maximal} bicliques in {r.exe_time_s:.3f} s", file=sys.stderr,
               flush=True)
-    results = []
+    f.set("count_bytes", 0)
+    f.set("root_begin", 0)
+    f.set("root_stride", K)
+    steps = []
+    parts = K if args.only <= 0 else min(K, args.only)
     t_start = time.perf_counter()
-    for i in range(args.steps):
+    for i in range(parts):
+        f.set("root_phase", i)
         r = f.execute()  # synchronous: ends with the counts on the host
-        results.append(r)
-        print(f"# step {i}: {r.maximal} bicliques, exe {r.exe_time_s:.3f} s, kernel "
-              f"{r.kernel_time_s:.3f} s", file=sys.stderr, flush=True)
+        sts = f.stats()
+        steps.append((r.maximal, r.hash_sum, r.hash_xor, r.nodes, r.kernel_time_s,
+                      sts["main_kernel_ns"] * 1e-9, sts["wide_kernel_ns"] * 1e-9, r.exe_time_s))
+        print(f"# step {i}: {r.maximal} bicliques, exe {r.exe_time_s:.3f} s, main kernel "
+              f"{sts['main_kernel_ns'] * 1e-9:.3f} s, wide pass {sts['wide_kernel_ns'] * 1e-9:.3f} s",
+              file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t_start
     f.close()
-    digests = {r.digest() for r in results}
-    if len(digests) != 1:
-        raise SystemExit(f"bench: steps disagree: {digests}")
-    r0 = results[0]
-    total = sum(r.maximal for r in results)
-    kernel_avg = sum(r.kernel_time_s for r in results) / len(results)
-    achieved = r0.alg_bytes / kernel_avg if kernel_avg > 0 else 0.0
-    workload = workload_name(args.config, args.scale, args.root_begin)
+    total = sum(x[0] for x in steps)
+    hsum = sum(x[1] for x in steps) % (1 << 64)
+    hxor = 0
+    for x in steps:
+        hxor ^= x[2]
+    nodes = sum(x[3] for x in steps)
+    main_avg = sum(x[5] for x in steps) / len(steps)
+    nodes_avg = nodes / len(steps)
+    recorded = None
+    if parts == K:
+        try:
+            with open(DIGEST_FILE) as fh:
+                d = json.load(fh)
+            recorded = [d["maximal"], int(d["hash_sum"], 16), int(d["hash_xor"], 16)] == \
+                [total, hsum, hxor]
+        except (OSError, ValueError, KeyError):
+            recorded = None
+    achieved = bytes_per_node * nodes_avg / main_avg if bytes_per_node and main_avg > 0 else 0.0
+    traffic = measured_traffic()
     line = {
         "metric": METRIC,
         "value": total / elapsed,
         "unit": "bicliques/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": parts,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": elapsed / parts * 1e3,
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic (seeded generator, built in-process)",
         "config": {
-            "workload": workload,
+            "workload": WORKLOAD if parts == K else WORKLOAD + f"; only the first {parts} of {K}",
             "U_size": st["U_size"], "V_size": st["V_size"], "edge_size": st["edge_size"],
-            "task_roots": st["V_size"] - min(args.root_begin, st["V_size"]),
-            "maximal_bicliques": int(r0.maximal),
-            "hash_sum": f"{r0.hash_sum:#018x}", "hash_xor": f"{r0.hash_xor:#018x}",
-            "non_maximal": int(r0.non_maximal), "nodes": int(r0.nodes),
-            "tasks": int(r0.tasks), "splits": int(r0.splits),
-            "exe_time_s": r0.exe_time_s, "kernel_time_s": kernel_avg,
+            "parts": K, "parts_timed": parts,
+            "maximal_bicliques": int(total), "hash_sum": f"{hsum:#018x}",
+            "hash_xor": f"{hxor:#018x}", "matches_recorded_digest": recorded,
+            "nodes": int(nodes), "main_kernel_s_per_launch": main_avg,
+            "wide_pass_s_total": sum(x[6] for x in steps),
+            "exe_time_s_total": sum(x[7] for x in steps),
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": measured_traffic(workload),
-            "alg_bytes_per_launch": int(r0.alg_bytes),
+            "frac": achieved / HBM_PEAK,
+            "traffic": traffic,
+            "traffic_bandwidth_GBs": (traffic / main_avg / 1e9) if traffic and main_avg else None,
+            "traffic_frac": (traffic / main_avg / HBM_PEAK) if traffic and main_avg else None,
+            "alg_bytes_per_node": bytes_per_node,
         },
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(graph, args.root_begin, args.cpu_seconds)
-        # the authors' improved CPU finder on the same sample, for comparison
-        line["cpu_baseline_imbea"] = cpu_baseline(graph, args.root_begin, args.cpu_seconds / 3,
-                                                  algo="imbea")
-        line["cpu_baseline_pmbe"] = cpu_baseline(graph, args.root_begin, args.cpu_seconds / 3,
-                                                 algo="pmbe")
+        line["cpu_baseline"] = cpu_baseline(graph, args.cpu_seconds)
+        g = gpu_on_cpu_sample(graph)
+        line["gpu_on_cpu_sample"] = {
+            "value": g.maximal / g.kernel_time_s if g.kernel_time_s > 0 else 0.0,
+            "unit": "bicliques/s", "kernel_time_s": g.kernel_time_s,
+            "digest_matches_cpu": line["cpu_baseline"]["complete"] and
+            [int(g.maximal), int(g.hash_sum), int(g.hash_xor)] == line["cpu_baseline"]["digest"],
+        }
+        line["cpu_baseline_imbea"] = cpu_baseline(graph, args.cpu_seconds / 3, algo="imbea")
+        line["cpu_baseline_pmbe"] = cpu_baseline(graph, args.cpu_seconds / 3, algo="pmbe")
     print(json.dumps(line), flush=True)
 
 

@@ -278,7 +278,7 @@ struct gmbe_finder {
   int64_t root_begin = 0, root_end = -1;  // processing-order rank range, -1 = all
   int64_t deadline_ms = 0;                 // 0 = run to completion
   int64_t late_share = 32;                 // 0 = off
-  int64_t dom_k = 8, dom_skip = 1;         // list reduction (reduce_list)
+  int64_t dom_k = 6, dom_skip = 1;         // list reduction (reduce_list)
   int64_t recompute = 0;                   // split large tasks without a free context into children
   int64_t count_bytes = 0;                 // count algorithmic bytes (alg_bytes)
   int64_t ctx_slots = 8;                   // shared-context slots per wave
@@ -310,7 +310,8 @@ struct gmbe_finder {
   uint64_t deferred_cap = 1ull << 20;
   uint64_t last_deferred = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr;  // evm: end of the main kernel
+  double main_ms = 0, wide_ms = 0;  // last execute: main kernel, wide-bitset pass
   bool ran = false;
   uint64_t list_used = 0, list_ovf = 0;
 };
@@ -327,7 +328,7 @@ static int finder_alloc(gmbe_finder* f) {
   const int64_t arena_kb = f->arena_set ? f->arena_kb
                                         : std::min<int64_t>(16384, std::max<int64_t>(4096, (int64_t)(nv * 24 * 64 / 1024)));
   const int64_t ctx_kb = f->ctx_set ? f->ctx_kb
-                                    : std::min<int64_t>(4096, std::max<int64_t>(512, (int64_t)(nv * 128 / 1024)));
+                                    : std::min<int64_t>(1024, std::max<int64_t>(256, (int64_t)(nv * 16 / 1024)));
   f->arena_kb = arena_kb;
   f->ctx_kb = ctx_kb;
   mbe::Scratch lay = scratch_layout(f->dg.nV, f->dg.dmaxV, (uint64_t)arena_kb * 1024ull,
@@ -348,8 +349,8 @@ static int finder_alloc(gmbe_finder* f) {
   f->layout = lay;
   f->nwaves = nwaves;
   f->nblocks = blocks;
-  uint64_t qcap = (1ull << f->queue_log2) / kShards;  // slots per shard
-  if (qcap < 16) qcap = 16;
+  uint64_t qcap = (1ull << f->queue_log2) / kShards;  // slots per shard (a power of two)
+  if (qcap < 1) qcap = 1;
   if (qcap != f->qcap) {
     if (f->q) (void)hipFree(f->q);
     if (f->qseq) (void)hipFree(f->qseq);
@@ -393,7 +394,8 @@ static int gmbe_finder_create_impl(const gmbe_graph* g, int sel, int bound_heigh
     return rc;
   }
   if (hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&f->ev0) != hipSuccess || hipEventCreate(&f->ev1) != hipSuccess) {
+      hipEventCreate(&f->ev0) != hipSuccess || hipEventCreate(&f->ev1) != hipSuccess ||
+      hipEventCreate(&f->evm) != hipSuccess) {
     delete f;
     return set_error(GMBE_ERR_HIP, "stream/event creation failed");
   }
@@ -443,7 +445,7 @@ static int gmbe_finder_set_impl(gmbe_finder* f, const char* key, int64_t value) 
       return set_error(GMBE_ERR_ARG, "lds_kb must be 0, 3, 7 or 16");
     f->lds_kb = value;
   }
-  else if (k == "queue_log2") { if ((rc = range(4, 30))) return rc; f->queue_log2 = value; }
+  else if (k == "queue_log2") { if ((rc = range(0, 30))) return rc; f->queue_log2 = value; }
   else if (k == "list_cap") { if ((rc = range(0, (int64_t)1 << 34))) return rc; f->list_cap = value; }
   else if (k == "arena_kb") { if ((rc = range(64, 1 << 20))) return rc; f->arena_kb = value; f->arena_set = true; }
   else if (k == "root_stride") { if ((rc = range(1, 1 << 30))) return rc; f->root_stride = value; }
@@ -562,6 +564,7 @@ static int gmbe_finder_execute_impl(gmbe_finder* f, gmbe_result* res) {
     hipLaunchKernelGGL(kc.fn, dim3(f->nblocks), dim3(kc.waves * 64), 0, f->stream, P);
     HIP_TRY(hipGetLastError());
   }
+  HIP_TRY(hipEventRecord(f->evm, f->stream));
   std::vector<unsigned long long> ctl_out(C_NSLOTS * kCtlStride, 0ull);
   HIP_TRY(hipMemcpyAsync(ctl_out.data(), f->ctl, ctl_out.size() * 8, hipMemcpyDeviceToHost,
                          f->stream));
@@ -597,8 +600,11 @@ static int gmbe_finder_execute_impl(gmbe_finder* f, gmbe_result* res) {
                          f->stream));
   HIP_TRY(hipStreamSynchronize(f->stream));
   double t1 = now_s();
-  float ms = 0;
+  float ms = 0, mms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, f->ev0, f->ev1));
+  HIP_TRY(hipEventElapsedTime(&mms, f->ev0, f->evm));
+  f->main_ms = mms;
+  f->wide_ms = f->last_deferred > 0 ? (double)ms - mms : 0.0;
   res->maximal = acc_h[mbe::A_MAXIMAL];
   res->non_maximal = acc_h[mbe::A_NONMAX];
   res->hash_sum = acc_h[mbe::A_HSUM];
@@ -703,8 +709,16 @@ int gmbe_finder_list(gmbe_finder* f, uint32_t* buf, size_t cap, size_t* used) {
 
 static int gmbe_finder_stats_impl(gmbe_finder* f, uint64_t* out, size_t n) {
   if (!f) return set_error(GMBE_ERR_ARG, "null finder");
-  const int first = mbe::A_CYC_WAIT, avail = mbe::A_COUNT - mbe::A_CYC_WAIT;
-  for (size_t i = 0; out && i < n && i < (size_t)avail; ++i) out[i] = f->prof[first + i];
+  // the profile counters, then the main kernel's and the wide pass's time (ns)
+  // and the tasks handed to the wide pass
+  const int first = mbe::A_CYC_WAIT, nprof = mbe::A_COUNT - mbe::A_CYC_WAIT;
+  uint64_t all[mbe::A_COUNT + 3];
+  for (int i = 0; i < nprof; ++i) all[i] = f->prof[first + i];
+  all[nprof] = (uint64_t)(f->main_ms * 1e6);
+  all[nprof + 1] = (uint64_t)(f->wide_ms * 1e6);
+  all[nprof + 2] = f->last_deferred;
+  const int avail = nprof + 3;
+  for (size_t i = 0; out && i < n && i < (size_t)avail; ++i) out[i] = all[i];
   return avail;
 }
 
@@ -728,6 +742,7 @@ void gmbe_finder_destroy(gmbe_finder* f) {
   (void)hipFree(f->deferred);
   if (f->ev0) (void)hipEventDestroy(f->ev0);
   if (f->ev1) (void)hipEventDestroy(f->ev1);
+  if (f->evm) (void)hipEventDestroy(f->evm);
   if (f->stream) (void)hipStreamDestroy(f->stream);
   delete f;
 }

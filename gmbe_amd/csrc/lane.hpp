@@ -35,7 +35,7 @@ struct LaneKnobs {
   int prune = 1, max_prefix = 4, group = 1, share = 1, split = 1;
   int64_t share_work = 4096, share_queue = -1, merge_min = 256, merge_branch = 16,
           late_share = 32;
-  int dom_k = 8, dom_skip = 1;  // list reduction (reduce_list)
+  int dom_k = 6, dom_skip = 1;  // list reduction (reduce_list)
   int recompute = 0;            // split large tasks without a free context into children
   int count_bytes = 0;          // count algorithmic bytes (alg_bytes)
   uint64_t deferred_cap = 1ull << 18;

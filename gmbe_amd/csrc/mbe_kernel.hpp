@@ -1250,8 +1250,8 @@ __device__ __forceinline__ uint32_t merge_twins(const WaveMem& m, Rec<W>* crec, 
 // later: a later dominator has no more bits and, if equal, a later lane).
 template <int W, bool LDS = false>
 __device__ __forceinline__ uint32_t reduce_chunk(const Params& P, Rec<W>* recs, uint32_t len, int K,
-                                 uint32_t minrep, uint64_t& best, uint32_t& cbranch,
-                                 Counters& ctr) {
+                                                 uint32_t minrep, uint64_t& best,
+                                                 uint32_t& cbranch, Counters& ctr) {
   const uint32_t lane = lane_id();
   const bool valid = lane < len;
   uint32_t mi = kNone, rp = kNone;
@@ -1283,30 +1283,48 @@ __device__ __forceinline__ uint32_t reduce_chunk(const Params& P, Rec<W>* recs, 
     for (int w = 0; w < W; ++w) pc += __popcll(a[w]);
     key = ((pc + 1) << 6) | (63u - lane);
   }
-  bool drop = false, sk = push && mi < rp;
+  // dominators, round by round: the excluded lane of largest key that is
+  // neither dropped nor chosen yet
+  bool drop = false, chosen = false, sk = P.dom_skip && push && mi < rp;
   for (int r = 0; r < K; ++r) {
-    const uint32_t mx = wave_max_u32(drop ? 0u : key);
+    const uint32_t mx = wave_max_u32(excl && !drop && !chosen ? key : 0u);
     if (mx == 0) break;
     const uint32_t pd = 63u - (mx & 63u);
-    bool inside = true, eq = true;
+    chosen = chosen || lane == pd;
+    bool inside = valid, eq = valid;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
       const uint64_t dw = read_lane64(a[w], pd);
       inside = inside && (a[w] & ~dw) == 0;
       eq = eq && a[w] == dw;
     }
-    if (inside) {
-      drop = drop || (excl && lane != pd && !(eq && lane < pd));
-      sk = sk || push;
-    }
-    if (lane == pd) key = 0;
+    // drop excluded lanes inside the dominator but itself and equal ones
+    // before it; mark every pushable lane inside it
+    drop = drop || (excl && inside && lane != pd && !(eq && lane < pd));
+    sk = sk || (push && inside);
   }
-  if (!P.dom_skip) sk = false;
-  const bool keep = valid && !drop;
-  const uint64_t km = ballot(keep);
+  const uint64_t vm = ballot(valid), pm = ballot(push), dm = ballot(drop);
+  uint64_t sm = ballot(sk);
+  if (!P.dom_skip) sm = 0;
+  const uint64_t km = vm & ~dm;
+  sm &= km;
+  const uint64_t bm = pm & km & ~sm;  // pushes left at this node
+  if (dm == 0 && sm == 0) {
+    // nothing dropped or marked: the list stands as it is
+    if (P.dom_skip) {
+      best = bm ? (((uint64_t)read_lane(rp, (uint32_t)__builtin_ctzll(bm)) << 32) |
+                   (uint32_t)__builtin_ctzll(bm))
+                : ~0ull;
+    } else {
+      best = wave_min_pair(lane_bit(bm) ? (((uint64_t)rp << 32) | lane) : ~0ull);
+    }
+    cbranch = __popcll(bm);
+    return len;
+  }
+  const bool keep = lane_bit(km);
+  sk = lane_bit(sm);
   const uint32_t pos = lanes_below(km);
   wave_mem_wait();  // every lane has read its entry before any is overwritten
-  uint64_t c = ~0ull;
   if (keep) {
     if constexpr (LDS) {
       auto* q = as_lds(recs) + pos;
@@ -1322,19 +1340,22 @@ __device__ __forceinline__ uint32_t reduce_chunk(const Params& P, Rec<W>* recs, 
 #pragma unroll
       for (int w = 0; w < W; ++w) recs[pos].a[w] = a[w];
     }
-    if (push && !sk) c = ((uint64_t)rp << 32) | pos;
   }
   wave_mem_wait();
-  const uint64_t bm = ballot(keep && push && !sk);
   if (P.dom_skip) {  // ordered list: the first push is the first such position
-    best = bm ? read_lane64(c, (uint32_t)__builtin_ctzll(bm)) : ~0ull;
+    if (bm) {
+      const uint32_t l = (uint32_t)__builtin_ctzll(bm);
+      best = ((uint64_t)read_lane(rp, l) << 32) | (uint32_t)__popcll(km & ((1ull << l) - 1ull));
+    } else {
+      best = ~0ull;
+    }
   } else {
-    best = wave_min_pair(c);
+    best = wave_min_pair(lane_bit(bm) ? (((uint64_t)rp << 32) | pos) : ~0ull);
   }
   cbranch = __popcll(bm);
   if (P.profile) {
-    ctr.dropped += __popcll(ballot(valid && drop));
-    ctr.skipped += __popcll(ballot(keep && sk));
+    ctr.dropped += __popcll(dm);
+    ctr.skipped += __popcll(sm);
   }
   return __popcll(km);
 }
@@ -1453,8 +1474,8 @@ __device__ __forceinline__ uint32_t reduce_list(const Params& P, const WaveMem& 
 // and large enough.  Returns that slot's id, or kNone.
 __device__ __forceinline__ uint32_t share_wanted(const Params& P, uint64_t gw, uint32_t nL,
                                                  uint32_t nC, uint32_t len, int W) {
-  const uint64_t bytes = sizeof(CtxHdr) + (uint64_t)len * (16 + 8 * W) + (uint64_t)nC * 4 +
-                         (uint64_t)nL * 8;
+  const uint64_t bytes = sizeof(CtxHdr) + (uint64_t)len * (16 + 8 * W) + (uint64_t)nL * 8 +
+                         (P.count_bytes ? (uint64_t)nC * 4 : 0);
   if (bytes > P.w.ctx_bytes) return kNone;
   if (uni64(ld_dev(P.s.root_next)) < roots_total(P.s)) return kNone;
   const long long depth = (long long)uni64(ld_dev(P.s.queued));
@@ -1646,12 +1667,22 @@ __device__ __forceinline__ void dfs(const Params& P, const WaveMem& m, uint32_t 
   const int64_t* voff = m.g->v_off;
   const uint32_t swapped = m.g->swapped;
   uint32_t depth = 0, tick = 0;
+  // this subtree's result counters, in scalar registers until it ends
+  uint64_t l_nodes = 0, l_nonmax = 0, l_max = 0, l_hsum = 0, l_hxor = 0;
+  auto flush = [&]() {
+    ctr.nodes += l_nodes;
+    ctr.nonmax += l_nonmax;
+    ctr.maximal += l_max;
+    ctr.hsum += l_hsum;
+    ctr.hxor ^= l_hxor;
+  };
   for (;;) {
     ++tick;
     // one wall-clock read per push: a push over a long list costs far more
     if (P.budget != 0 && past_deadline(P, m.deadline)) {
       ctr.abandoned++;
       graph_work(P, graph_id(P, m), 1);  // never finished: the graph stays incomplete
+      flush();
       return;
     }
     if (F.cur == kNone) {
@@ -1689,6 +1720,7 @@ __device__ __forceinline__ void dfs(const Params& P, const WaveMem& m, uint32_t 
       // no room for the child list: report and abandon the task
       ctr.overflow++;
       ctr.ovf_arena++;
+      flush();
       return;
     }
     R_t* crec = (R_t*)crec_addr;
@@ -1709,10 +1741,12 @@ __device__ __forceinline__ void dfs(const Params& P, const WaveMem& m, uint32_t 
     }
     if (P.profile) ctr.c_pass += cycles() - tp;
     ctr.nodes++;
-    ctr.entries += F.len;
-    if (W == 1) {
-      ctr.ent_w1 += F.len;
-      ctr.nodes_w1++;
+    if (P.profile) {
+      ctr.entries += F.len;
+      if (W == 1) {
+        ctr.ent_w1 += F.len;
+        ctr.nodes_w1++;
+      }
     }
     F.cur = o.best_next == ~0ull ? kNone : (uint32_t)o.best_next;
     F.curidx = (uint32_t)(o.best_next >> 32);
@@ -1797,6 +1831,7 @@ __device__ __forceinline__ void dfs(const Params& P, const WaveMem& m, uint32_t 
       F.top = (uint32_t)ctop;
     }
   }
+  flush();
 }
 
 // Run the items of one entry (`word`: lane l holds word l).
@@ -1986,7 +2021,9 @@ __device__ __forceinline__ void enumerate_task(const Params& P, const WaveMem& m
     return;
   }
   TaskArrays lta{m.kL, m.C};
-  if (share_ctx != kNone &&
+  const uint64_t need = sizeof(CtxHdr) + (uint64_t)len * sizeof(R_t) + (uint64_t)nL * 8 +
+                        (P.count_bytes ? (uint64_t)nC * 4 : 0);
+  if (share_ctx != kNone && need <= P.w.ctx_bytes &&
       publish_list<Wt, false, false>(P, gw, share_ctx, graph_id(P, m), (const R_t*)list_addr, len, nL, nC, nL,
                                      sumR, lta)) {
     ctr.splits++;
@@ -2234,11 +2271,8 @@ __device__ __forceinline__ void run_task(const Params& P, const WaveMem& m, cons
   // large task: share its root list with other waves when one of this wave's
   // context slots is free and large enough
   uint32_t share_ctx = kNone;
-  if (rule && P.share && !P.list && W <= (uint32_t)WMAX) {
-    const uint32_t Wt = W <= 1 ? 1 : (W <= 2 ? 2 : (W <= 4 ? 4 : (W <= 8 ? 8 : 16)));
-    const uint64_t bytes = sizeof(CtxHdr) + (uint64_t)nC * (16 + 8 * Wt + 12) + (uint64_t)nL * 8;
-    if (bytes <= P.w.ctx_bytes) share_ctx = free_ctx(P, gw);
-  }
+  // (whether its reduced root list fits the slot is decided once it is built)
+  if (rule && P.share && !P.list && W <= (uint32_t)WMAX) share_ctx = free_ctx(P, gw);
   const bool share = share_ctx != kNone;
   // a large task with no context slot free: its children are recomputed by
   // other waves (recompute, the reference's LargeToTiny) or, by default, the

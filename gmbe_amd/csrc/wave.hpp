@@ -70,6 +70,12 @@ __device__ __forceinline__ uint32_t wave_incl_scan32(uint32_t x) {
   return x;
 }
 __device__ __forceinline__ int first_lane(uint64_t m) { return __builtin_ctzll(m); }
+// This lane's bit of a uniform lane mask (32-bit shifts only).
+__device__ __forceinline__ bool lane_bit(uint64_t m) {
+  const uint32_t lane = lane_id();
+  const uint32_t w = lane < 32u ? (uint32_t)m : (uint32_t)(m >> 32);
+  return (w >> (lane & 31u)) & 1u;
+}
 
 // Maximum over the wave by DPP: row_shr 1/2/4/8 leave each row's maximum in
 // its last lane, row_bcast:15 and row_bcast:31 carry rows into lane 63.

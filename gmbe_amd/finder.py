@@ -61,11 +61,12 @@ class IterFinderGpu6:
                   "cyc_split", "edges", "entries", "ovf_arena", "ovf_wide", "ovf_defer",
                   "entries_w1", "nodes_w1", "cyc_tail", "sub_pass", "sub_merge",
                   "sub_compress", "sub_publish", "publishes", "merges", "abandoned", "late",
-                  "sub_reduce", "dropped", "skipped"]
+                  "sub_reduce", "dropped", "skipped", "main_kernel_ns", "wide_kernel_ns",
+                  "wide_tasks"]
 
     def stats(self) -> dict:
-        out = (C.c_uint64 * 32)()
-        n = lib().gmbe_finder_stats(self._h, out, 32)
+        out = (C.c_uint64 * 64)()
+        n = lib().gmbe_finder_stats(self._h, out, 64)
         return {k: int(out[i]) for i, k in enumerate(self.STAT_NAMES[:n])}
 
     def print_result(self):
@@ -109,3 +110,4 @@ def enumerate_graph(graph: BiGraph, bound_height=20, bound_size=1500, sel=2, **k
         return f.execute()
     finally:
         f.close()
+

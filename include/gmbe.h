@@ -230,8 +230,12 @@ int gmbe_finder_exit_times(const gmbe_finder* f, double* out, size_t cap, size_t
  * [15..18] cycles inside the stack machine spent in push passes, twin
  * merges, bitset re-indexing and context publishing; [19] contexts
  * published, [20] twin merges; [21] subtrees abandoned at the deadline;
- * [22] frames handed over by late sharing.
- * Returns the number of counters available. */
+ * [22] frames handed over by late sharing; [23] cycles spent reducing
+ * lists, [24] excluded entries dropped, [25] pushes marked non-maximal;
+ * [26] main kernel time and [27] wide-bitset pass time of the last execute
+ * (ns, HIP events on the finder's stream), [28] tasks handed to the wide pass.
+ * Counters [0..25] are filled when the knob "profile" is 1 (else 0); [26..28]
+ * always.  Returns the number of counters available. */
 int gmbe_finder_stats(gmbe_finder* f, uint64_t* out, size_t n);
 
 /* Replaces the finder destructor chain (src/IterFinderGpu.cu:1087-1093). */

@@ -54,3 +54,21 @@ def test_cli_refuses_several_gpus(adj):
 def test_cli_missing_file_is_an_error(tmp_path):
     r = run("-i", str(tmp_path / "none.adj"), "-s", "2")
     assert r.returncode != 0 and "can't open" in r.stderr
+
+
+@pytest.mark.gpu
+def test_cli_count_on_the_gpu(gpu, tmp_path):
+    """The drop-in CLI on the uniform 2,000 x 1,500 graph: the reference's
+    stdout lines, in the order its scripts read them, with the oracle's count."""
+    import oracle
+    from gmbe_amd import workloads
+    raw = workloads.raw_graph(1)
+    path = str(tmp_path / "c1.adj")
+    raw.write_adj(path)
+    expect = oracle.gmbe(raw.derive(**workloads.RUN_OPTION).oracle_graph()).maximal
+    r = run("-i", path, "-s", "2", "-t", "1", "-o", "1", "-f")
+    assert r.returncode == 0, r.stderr
+    out = r.stdout
+    assert f"maximal nodes:{expect}" in out
+    assert out.index("non_maximal:") < out.index("Total processing time") < \
+        out.index("maximal nodes:")

@@ -68,11 +68,9 @@ def test_looped_baseline_matches(gpu, small_set, expected):
     {"bound_height.0": 0, "bound_size.0": 0, "bound_height.1": 0, "bound_size.1": 0,
      "bound_height.2": 0, "bound_size.2": 0, "bound_height.3": 0, "bound_size.3": 0},
     {"share": 0}, {"lds_kb": 0}, {"lds_kb": 16}, {"waves_per_block": 8},
-    {"waves_per_block": 2}, {"group": 0},
+    {"waves_per_block": 2}, {"group": 0}, {"split": 0},
 ])
 def test_knobs_keep_per_graph_results(gpu, small_set, expected, knobs):
-    # (split=0 is left out: these graphs have hub roots whose unsplit subtrees
-    # run for minutes on one wave; the unsplit traversal is checked on config 1)
     b = Batch(small_set)
     for k, v in knobs.items():
         b.set(k, v)
@@ -82,10 +80,8 @@ def test_knobs_keep_per_graph_results(gpu, small_set, expected, knobs):
 
 
 def test_autotuner_keeps_results(gpu, small_set, expected):
-    # graphs of every size class but the hub-heavy ones, so that the tuning
-    # trials stay short
-    pick = [i for i, g in enumerate(small_set) if g.stats()["edge_size"] < 16000]
-    gs = [small_set[i] for i in pick]
+    pick = list(range(len(small_set)))
+    gs = small_set
     b = Batch(gs)
     b.set("autotune", 1)
     assert b.get("waves_per_block") in (2, 4, 8)

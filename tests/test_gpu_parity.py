@@ -133,17 +133,35 @@ def test_edge_cases(gpu):
 
 
 def test_capacity_error_is_reported(gpu):
-    """A per-wave arena too small for a task is an error, never a silent miss."""
+    """A per-wave arena too small for a task is an error, never a silent miss:
+    on the 2% power-law graph some top-level vertex has more than 64 KB of
+    candidate records (over 2,730 candidates at 24 bytes)."""
     g = workloads.run_graph(2, 0.02)
     f = IterFinderGpu6(g, arena_kb=64, lds_kb=0)
-    try:
-        r = f.execute()
-    except GmbeError as e:
-        assert e.code == -4
-    else:
-        assert r.overflow == 0
-        assert r.maximal == REF["config2_s0.02"]["count"]
+    with pytest.raises(GmbeError) as e:
+        f.execute()
+    assert e.value.code == -4
+    assert f.result.overflow > 0
     f.close()
+
+
+def test_full_queue_falls_back_to_local_runs(gpu):
+    """A queue of one slot per shard (32 in all) with every task split and no
+    shared contexts: pushes that find no room run the task locally instead,
+    and the result is the oracle's."""
+    g = workloads.run_graph(1)
+    base = oracle.gmbe(g.oracle_graph()).digest()
+    r = gpu_digest(g, queue_log2=0, share=0, bound_height=0, bound_size=0)
+    assert r.digest() == base
+
+
+@pytest.mark.parametrize("waves", [2, 4, 8])
+def test_waves_per_block(gpu, waves):
+    """Two, four or eight waves per block (the reference's warps-per-block
+    builds, src/CMakeLists.txt:57-78) give the oracle's result."""
+    g = workloads.run_graph(2, 0.01)
+    base = oracle.gmbe(g.oracle_graph(), threads=8).digest()
+    assert gpu_digest(g, waves_per_block=waves).digest() == base
 
 
 def test_repeated_execute_is_identical(gpu):

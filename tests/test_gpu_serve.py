@@ -1,9 +1,11 @@
 """Serving loop: every request's count and hash equal the plain finder's on
 the same request graph, whether it ran alone or in a mix of overlapping
-launches; admission and error paths."""
+launches, and the oracle's on a sample; admission and error paths.  The
+resident graph is the full-size power-law graph of the served workload."""
 import numpy as np
 import pytest
 
+import oracle
 from gmbe_amd import GmbeError, Server, enumerate_graph, workloads
 
 pytestmark = pytest.mark.gpu
@@ -12,7 +14,7 @@ RUN = workloads.RUN_OPTION
 
 @pytest.fixture(scope="module")
 def resident():
-    return workloads.raw_graph(2, 0.1)
+    return workloads.raw_graph(2)
 
 
 def alone(g_raw):
@@ -49,6 +51,29 @@ def test_mixed_requests_equal_alone(gpu, resident):
         assert d.status == 0
         assert (d.maximal, d.hash_sum, d.hash_xor) == want[t]
         assert d.t_done_s >= d.t_submit_s
+
+
+def test_served_requests_equal_the_oracle(gpu, resident):
+    """Ego subgraphs of the resident graph and small graphs, served in one
+    mix, against the CPU oracle bit for bit."""
+    rng = np.random.default_rng(11)
+    nU = resident.stats()["U_size"]
+    s = Server(resident, slots=16, mem_budget=1 << 28)
+    want = {}
+    for j in range(8):
+        if j % 2 == 0:
+            u = int(rng.integers(0, nU))
+            t = s.submit_ego(u, 20000)
+            g = resident.ego(u, 20000)
+        else:
+            g = workloads.small_graph(workloads.serve_small_seed(200 + j))
+            t = s.submit_graph(g)
+        want[t] = oracle.gmbe(g.derive(**RUN).oracle_graph(), threads=8).digest()
+    got = drain(s, len(want))
+    s.close()
+    for t, d in got.items():
+        assert d.status == 0
+        assert (d.maximal, d.hash_sum, d.hash_xor) == want[t]
 
 
 def test_tight_budget_admits_one_at_a_time(gpu, resident):

@@ -77,17 +77,24 @@ def test_bench_cpu_baseline_legs():
     import bench
     import oracle
     g = workloads.run_graph(1)
-    total = oracle.mbea(g.oracle_graph(), threads=4).maximal
-    for algo in ("mbea", "imbea", "pmbe"):
-        b = bench.cpu_baseline(g, 0, 30.0, algo=algo)
-        assert set(b) == {"value", "unit", "cores", "kind", "sample"}
-        assert b["unit"] == "bicliques/s" and b["kind"] == "port" and b["cores"] >= 1
-        assert b["value"] > 0
-        # stride 97 from rank 0: MBEA and iMBEA walk all 97 phases, PMBE one
-        if algo != "pmbe":
+    full = oracle.mbea(g.oracle_graph(), threads=4)
+    total = full.maximal
+    first, stride = bench.CPU_FIRST, bench.CPU_STRIDE
+    try:
+        # the whole graph as the sample: rank 0, stride 1
+        bench.CPU_FIRST, bench.CPU_STRIDE = 0, 1
+        for algo in ("mbea", "imbea", "pmbe"):
+            b = bench.cpu_baseline(g, 30.0, algo=algo)
+            assert set(b) == {"value", "unit", "cores", "kind", "cpu", "sample", "complete",
+                              "digest"}
+            assert b["unit"] == "bicliques/s" and b["kind"] == "port" and b["cores"] >= 1
+            assert b["value"] > 0 and b["complete"]
             assert f"{total} maximal bicliques" in b["sample"], b["sample"]
-        else:
-            assert "containment order" in b["sample"]
+            assert b["digest"] == [total, full.hash_sum, full.hash_xor]
+            if algo == "pmbe":
+                assert "containment order" in b["sample"]
+    finally:
+        bench.CPU_FIRST, bench.CPU_STRIDE = first, stride
 
 
 def test_finder_knob_errors_are_codes():
