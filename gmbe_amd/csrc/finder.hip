@@ -312,6 +312,7 @@ struct gmbe_finder {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr;  // evm: end of the main kernel
   double main_ms = 0, wide_ms = 0;  // last execute: main kernel, wide-bitset pass
+  unsigned long long main_acc[mbe::A_COUNT] = {};  // counters after the main kernel
   bool ran = false;
   uint64_t list_used = 0, list_ovf = 0;
 };
@@ -350,7 +351,9 @@ static int finder_alloc(gmbe_finder* f) {
   f->nwaves = nwaves;
   f->nblocks = blocks;
   uint64_t qcap = (1ull << f->queue_log2) / kShards;  // slots per shard (a power of two)
-  if (qcap < 1) qcap = 1;
+  // at least two: a slot's sequence number tells "holds position p" (p + 1)
+  // from "free for position p + qcap" (p + qcap) only when qcap >= 2
+  if (qcap < 2) qcap = 2;
   if (qcap != f->qcap) {
     if (f->q) (void)hipFree(f->q);
     if (f->qseq) (void)hipFree(f->qseq);
@@ -568,6 +571,8 @@ static int gmbe_finder_execute_impl(gmbe_finder* f, gmbe_result* res) {
   std::vector<unsigned long long> ctl_out(C_NSLOTS * kCtlStride, 0ull);
   HIP_TRY(hipMemcpyAsync(ctl_out.data(), f->ctl, ctl_out.size() * 8, hipMemcpyDeviceToHost,
                          f->stream));
+  HIP_TRY(hipMemcpyAsync(f->main_acc, f->acc, sizeof(f->main_acc), hipMemcpyDeviceToHost,
+                         f->stream));
   HIP_TRY(hipStreamSynchronize(f->stream));
   f->last_deferred = std::min<uint64_t>(ctl_out[C_DEFERRED * kCtlStride], f->deferred_cap);
   // the wide pass gets what is left of the budget
@@ -709,15 +714,18 @@ int gmbe_finder_list(gmbe_finder* f, uint32_t* buf, size_t cap, size_t* used) {
 
 static int gmbe_finder_stats_impl(gmbe_finder* f, uint64_t* out, size_t n) {
   if (!f) return set_error(GMBE_ERR_ARG, "null finder");
-  // the profile counters, then the main kernel's and the wide pass's time (ns)
-  // and the tasks handed to the wide pass
+  // the profile counters, then the main kernel's and the wide pass's time (ns),
+  // the tasks handed to the wide pass, and the nodes and maximal bicliques of
+  // the main kernel alone
   const int first = mbe::A_CYC_WAIT, nprof = mbe::A_COUNT - mbe::A_CYC_WAIT;
-  uint64_t all[mbe::A_COUNT + 3];
+  uint64_t all[mbe::A_COUNT + 5];
   for (int i = 0; i < nprof; ++i) all[i] = f->prof[first + i];
   all[nprof] = (uint64_t)(f->main_ms * 1e6);
   all[nprof + 1] = (uint64_t)(f->wide_ms * 1e6);
   all[nprof + 2] = f->last_deferred;
-  const int avail = nprof + 3;
+  all[nprof + 3] = f->main_acc[mbe::A_NODES];
+  all[nprof + 4] = f->main_acc[mbe::A_MAXIMAL];
+  const int avail = nprof + 5;
   for (size_t i = 0; out && i < n && i < (size_t)avail; ++i) out[i] = all[i];
   return avail;
 }

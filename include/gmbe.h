@@ -185,13 +185,19 @@ int gmbe_finder_create(const gmbe_graph* g, int sel, int bound_height,
 
 /* Tuning knobs, set before execute (the reference's compile-time macros,
  * src/CMakeLists.txt:13-39, src/GpuUtil.cuh:9-24, src/IterFinderGpu.cu:912-917):
- *   "waves_per_block", "blocks", "queue_log2", "list_cap" (u32 words of list
- *   output; 0 = off), "prune" (0/1, PRUNE_EN), "max_prefix" (2..8),
- *   "bound_height", "bound_size".
- * Scheduler and memory knobs (no counterpart): "lds_kb", "arena_kb" and
- *   "ctx_kb" (per wave; unless set they scale with |V|: 4-16 MB and
- *   0.5-4 MB), "ctx_slots", "group", "share", "share_work", "share_queue", "merge_min",
- *   "merge_branch", "late_share", "profile".
+ *   "waves_per_block" (2, 4 or 8), "blocks", "queue_log2" (0..30: 2^queue_log2
+ *   queue entries over 32 shards, at least two per shard), "list_cap" (u32
+ *   words of list output; 0 = off), "prune" (0/1, PRUNE_EN), "max_prefix"
+ *   (2..8), "bound_height", "bound_size".
+ * Scheduler and memory knobs (no counterpart): "lds_kb" (LDS list arena per
+ *   wave: 0, 3, 7 or 16; default 7), "arena_kb" and "ctx_kb" (per wave;
+ *   unless set they scale with |V|: 4-16 MB and 256 KB-1 MB), "ctx_slots",
+ *   "group", "share", "share_work", "share_queue", "merge_min",
+ *   "merge_branch", "late_share", "profile", "dom_k" (0..16 dominators per
+ *   list reduction; 0 = off), "dom_skip" (0/1: mark pushes inside a
+ *   dominator non-maximal), "recompute" (0/1: split a large task with no
+ *   free context into recomputed children), "count_bytes" (0/1: fill
+ *   gmbe_result.alg_bytes).
  * "deadline_ms" (0 = none): every wave stops taking work, and abandons the
  *   subtree it is in, once it has run that long; execute then returns
  *   GMBE_ERR_DEADLINE with the partial counts filled in.
@@ -233,9 +239,11 @@ int gmbe_finder_exit_times(const gmbe_finder* f, double* out, size_t cap, size_t
  * [22] frames handed over by late sharing; [23] cycles spent reducing
  * lists, [24] excluded entries dropped, [25] pushes marked non-maximal;
  * [26] main kernel time and [27] wide-bitset pass time of the last execute
- * (ns, HIP events on the finder's stream), [28] tasks handed to the wide pass.
- * Counters [0..25] are filled when the knob "profile" is 1 (else 0); [26..28]
- * always.  Returns the number of counters available. */
+ * (ns, HIP events on the finder's stream), [28] tasks handed to the wide pass,
+ * [29] nodes and [30] maximal bicliques of the main kernel alone (the rest
+ * are the wide pass's).  The cycle counters ([0..6], [14..18], [23]) and the
+ * list sizes [8], [12], [13] are filled when the knob "profile" is 1 (else 0);
+ * the others always.  Returns the number of counters available. */
 int gmbe_finder_stats(gmbe_finder* f, uint64_t* out, size_t n);
 
 /* Replaces the finder destructor chain (src/IterFinderGpu.cu:1087-1093). */

@@ -21,7 +21,8 @@ class OracleResult(C.Structure):
         ("maximal", C.c_uint64), ("non_maximal", C.c_uint64),
         ("hash_sum", C.c_uint64), ("hash_xor", C.c_uint64),
         ("nodes", C.c_uint64), ("b_alg", C.c_uint64), ("tasks", C.c_uint64),
-        ("splits", C.c_uint64), ("skipped", C.c_uint64), ("seconds", C.c_double),
+        ("splits", C.c_uint64), ("skipped", C.c_uint64), ("cut", C.c_uint64),
+        ("seconds", C.c_double),
     ]
 
     def digest(self):

@@ -69,6 +69,7 @@ typedef struct {
   uint64_t maximal, non_maximal, hash_sum, hash_xor, nodes, b_alg, tasks;
   uint64_t splits;  /* orc_gmbe: tasks split by the bound rule */
   uint64_t skipped; /* orc_pmbe: candidates skipped inside a containment range */
+  uint64_t cut;     /* threads whose search the time limit cut (counts partial) */
   double seconds;
 } orc_result;
 
@@ -1299,6 +1300,7 @@ static void* run_worker(void* arg) {
       }
     }
     wk->res = w.res;
+    wk->res.cut = (uint64_t)w.cut;
     wk->list = w.list;
     free(w.cmap);
     free(w.touched);
@@ -1358,6 +1360,7 @@ static int run(int kind, uint32_t nU, uint32_t nV, const int64_t* v_off, const u
     out->tasks += r->tasks;
     out->splits += r->splits;
     out->skipped += r->skipped;
+    out->cut += r->cut;
     if (wk[i].rc) err = wk[i].rc;
     if (wk[i].list) {
       if (list) list_append(list, wk[i].list);

@@ -146,9 +146,12 @@ def test_capacity_error_is_reported(gpu):
 
 
 def test_full_queue_falls_back_to_local_runs(gpu):
-    """A queue of one slot per shard (32 in all) with every task split and no
-    shared contexts: pushes that find no room run the task locally instead,
-    and the result is the oracle's."""
+    """The smallest queue (two slots per shard, 64 in all) with every task
+    split and no shared contexts: pushes that find no room run the task
+    locally instead, and the result is the oracle's.  (A shard of one slot
+    would make a slot's "holds position p" and "free for position p + 1"
+    sequence numbers equal, so a pusher could overwrite an entry before its
+    popper read it; the finder rounds the capacity up to two.)"""
     g = workloads.run_graph(1)
     base = oracle.gmbe(g.oracle_graph()).digest()
     r = gpu_digest(g, queue_log2=0, share=0, bound_height=0, bound_size=0)
