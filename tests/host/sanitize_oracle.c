@@ -9,7 +9,7 @@
 #include <string.h>
 
 typedef struct {
-  uint64_t maximal, non_maximal, hash_sum, hash_xor, nodes, b_alg, tasks, splits, skipped;
+  uint64_t maximal, non_maximal, hash_sum, hash_xor, nodes, b_alg, tasks, splits, skipped, cut;
   double seconds;
 } orc_result;
 typedef struct orc_list orc_list;
