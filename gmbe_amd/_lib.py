@@ -82,6 +82,7 @@ SIGNATURES = {
     "gmbe_finder_create": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _PP]),
     "gmbe_finder_set": (C.c_int, [_P, C.c_char_p, C.c_int64]),
     "gmbe_finder_execute": (C.c_int, [_P, C.POINTER(Result)]),
+    "gmbe_finder_execute_static": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(Result)]),
     "gmbe_finder_exit_times": (C.c_int, [_P, C.POINTER(C.c_double), C.c_size_t, C.POINTER(C.c_size_t)]),
     "gmbe_finder_list": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "gmbe_finder_stats": (C.c_int, [_P, C.POINTER(C.c_uint64), C.c_size_t]),

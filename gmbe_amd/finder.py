@@ -111,3 +111,12 @@ def enumerate_graph(graph: BiGraph, bound_height=20, bound_size=1500, sel=2, **k
     finally:
         f.close()
 
+
+def execute_static(graph: BiGraph, bound_height=20, bound_size=1500) -> Result:
+    """-s 2 enumeration through gmbe_finder_execute_static: no device memory
+    is allocated; every buffer comes out of the library's static 128 MiB
+    array.  Calls that overlap on one GPU race (the array is shared)."""
+    r = Result()
+    check(lib().gmbe_finder_execute_static(graph.handle, bound_height, bound_size,
+                                           C.byref(r)))
+    return r

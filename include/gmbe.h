@@ -211,6 +211,22 @@ int gmbe_finder_set(gmbe_finder* f, const char* key, int64_t value);
  * synchronous; exe_time_s is measured like exe_time_. */
 int gmbe_finder_execute(gmbe_finder* f, gmbe_result* res);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Runs the -s 2 enumeration of `g` like
+ * gmbe_finder_create + gmbe_finder_execute with the default knobs, but with
+ * every device buffer (graph copy, control block, queue, per-wave slabs)
+ * carved out of one static device array of 128 MiB that the library holds on
+ * each GPU it runs on until the process ends: no hipMalloc, hipFree or other
+ * allocating call.  The grid is sized to what fits the array (at most 256
+ * waves).  A graph whose copy and one block of slabs do not fit returns
+ * GMBE_ERR_CAPACITY before anything runs; a run whose lists outgrow the static
+ * arena returns GMBE_ERR_CAPACITY with the partial counts.  Synchronous, on
+ * the null stream.  Every call on a GPU shares its array, so calls that
+ * overlap on one GPU (from several host threads) race: serialize them. */
+int gmbe_finder_execute_static(const gmbe_graph* g, int bound_height, int bound_size,
+                               gmbe_result* res);
+
 /* List mode (no reference counterpart: the reference never emits pairs).
  * After an execute with list_cap > 0, copies records
  *   [nA, nB, A[0..nA), B[0..nB)]   (original ids, A on the original U side)
