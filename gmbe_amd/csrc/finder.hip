@@ -281,6 +281,7 @@ struct gmbe_finder {
   int64_t dom_k = 6, dom_skip = 1;         // list reduction (reduce_list)
   int64_t recompute = 0;                   // split large tasks without a free context into children
   int64_t count_bytes = 0;                 // count algorithmic bytes (alg_bytes)
+  int64_t reduce_min = 0;                  // reduce only lists at least this long
   int64_t ctx_slots = 8;                   // shared-context slots per wave
   int64_t profile = 0, group = 1, share = 1, ctx_kb = 4096, share_work = 4096, share_queue = -1;
   int64_t merge_min = 256, merge_branch = 64;
@@ -467,6 +468,7 @@ static int gmbe_finder_set_impl(gmbe_finder* f, const char* key, int64_t value) 
   else if (k == "dom_skip") { if ((rc = range(0, 1))) return rc; f->dom_skip = value; }
   else if (k == "recompute") { if ((rc = range(0, 1))) return rc; f->recompute = value; }
   else if (k == "count_bytes") { if ((rc = range(0, 1))) return rc; f->count_bytes = value; }
+  else if (k == "reduce_min") { if ((rc = range(0, 1 << 20))) return rc; f->reduce_min = value; }
   else if (k == "deadline_ms") { if ((rc = range(0, 1ll << 32))) return rc; f->deadline_ms = value; }
   else if (k == "root_begin") { if ((rc = range(0, (1ll << 32)))) return rc; f->root_begin = value; }
   else if (k == "root_end") { if ((rc = range(-1, (1ll << 32)))) return rc; f->root_end = value; }
@@ -556,6 +558,7 @@ static int gmbe_finder_execute_impl(gmbe_finder* f, gmbe_result* res) {
   P.dom_skip = (int)f->dom_skip;
   P.recompute = (int)f->recompute;
   P.count_bytes = (int)f->count_bytes;
+  P.reduce_min = (int)f->reduce_min;
   P.share_work = (int)std::min<int64_t>(f->share_work, INT32_MAX);
   P.share_queue = f->share_queue >= 0 ? (int)f->share_queue : f->nwaves;
   KernelChoice kc = pick_kernel((int)f->lds_kb, (int)f->waves_per_block, false);
@@ -715,17 +718,18 @@ int gmbe_finder_list(gmbe_finder* f, uint32_t* buf, size_t cap, size_t* used) {
 static int gmbe_finder_stats_impl(gmbe_finder* f, uint64_t* out, size_t n) {
   if (!f) return set_error(GMBE_ERR_ARG, "null finder");
   // the profile counters, then the main kernel's and the wide pass's time (ns),
-  // the tasks handed to the wide pass, and the nodes and maximal bicliques of
-  // the main kernel alone
+  // the tasks handed to the wide pass, and the nodes, maximal bicliques and
+  // algorithmic bytes of the main kernel alone
   const int first = mbe::A_CYC_WAIT, nprof = mbe::A_COUNT - mbe::A_CYC_WAIT;
-  uint64_t all[mbe::A_COUNT + 5];
+  uint64_t all[mbe::A_COUNT + 6];
   for (int i = 0; i < nprof; ++i) all[i] = f->prof[first + i];
   all[nprof] = (uint64_t)(f->main_ms * 1e6);
   all[nprof + 1] = (uint64_t)(f->wide_ms * 1e6);
   all[nprof + 2] = f->last_deferred;
   all[nprof + 3] = f->main_acc[mbe::A_NODES];
   all[nprof + 4] = f->main_acc[mbe::A_MAXIMAL];
-  const int avail = nprof + 5;
+  all[nprof + 5] = f->main_acc[mbe::A_BALG];
+  const int avail = nprof + 6;
   for (size_t i = 0; out && i < n && i < (size_t)avail; ++i) out[i] = all[i];
   return avail;
 }

@@ -143,6 +143,7 @@ struct Params {
   int dom_skip;              // mark pushes a dominator shows non-maximal (reduce_list)
   int recompute;             // a large task with no free context slot: split into recomputed children
   int count_bytes;           // count the algorithmic bytes of every node (alg_bytes)
+  int reduce_min;            // reduce only lists at least this long
   uint64_t budget;           // wall-clock ticks a wave may run (0 = no deadline)
   unsigned long long* exit_stamps;  // if set: wall clock at which each block's waves were done
 };
@@ -917,6 +918,7 @@ __device__ __forceinline__ uint64_t key_sum_of_bits(const WaveMem& m, const uint
 template <int W>
 __device__ __forceinline__ void compress_list(const Rec<W>* src, uint32_t len, const uint64_t* ax, Rec<1>* dst) {
   const uint32_t lane = lane_id();
+  lds_flat_fence();  // src / dst may be LDS lists reached by ds_ elsewhere
   for (uint32_t i = lane; i < len; i += kWave) {
     uint64_t out = 0;
     uint32_t t = 0;
@@ -936,7 +938,7 @@ __device__ __forceinline__ void compress_list(const Rec<W>* src, uint32_t len, c
     dst[i].ksum = src[i].ksum;
     dst[i].a[0] = out;
   }
-  wave_mem_wait();
+  lds_flat_fence();
 }
 
 // ---------------------------------------------------------------------------
@@ -1013,6 +1015,7 @@ __device__ __forceinline__ bool publish_list(const Params& P, uint64_t gw, uint3
   Rec<W>* recs;
   TaskArrays ta;
   ctx_arrays<W>(h, nL, nC, len, recs, ta);
+  lds_flat_fence();  // `root` and src.kL may be in LDS, written with ds_ stores
   // items: pushable groups, in the order of the list
   uint32_t n_items = 0;
   for (uint32_t base = 0; base < len; base += kWave) {
@@ -1117,9 +1120,10 @@ __device__ __forceinline__ bool publish_list(const Params& P, uint64_t gw, uint3
 // pushable count; leaves the list unchanged (returns len) when the arenas
 // lack room for the table.
 template <int W>
-__device__ __forceinline__ uint32_t merge_twins(const WaveMem& m, Rec<W>* crec, uint32_t len, uint32_t& lds_top,
-                                uint64_t& hbm_top, uint64_t& best_child, uint32_t& cbranch,
-                                bool ordered) {
+__device__ __forceinline__ uint32_t merge_twins_impl(const WaveMem& m, Rec<W>* crec, uint32_t len,
+                                                     uint32_t& lds_top, uint64_t& hbm_top,
+                                                     uint64_t& best_child, uint32_t& cbranch,
+                                                     bool ordered) {
   const uint32_t lane = lane_id();
   struct Slot {
     unsigned long long id;  // hash << 32 | (first-inserted entry + 1); 0 = empty
@@ -1222,6 +1226,21 @@ __device__ __forceinline__ uint32_t merge_twins(const WaveMem& m, Rec<W>* crec, 
   wave_mem_wait();
   best_child = (wave_min_pair(best));
   cbranch = nbr;
+  return n;
+}
+
+// The list and the table are reached through generic pointers (flat_
+// accesses) while an LDS list is read and written with ds_ accesses
+// elsewhere: fenced on both sides.
+template <int W>
+__device__ __forceinline__ uint32_t merge_twins(const WaveMem& m, Rec<W>* crec, uint32_t len,
+                                                uint32_t& lds_top, uint64_t& hbm_top,
+                                                uint64_t& best_child, uint32_t& cbranch,
+                                                bool ordered) {
+  lds_flat_fence();
+  const uint32_t n = merge_twins_impl<W>(m, crec, len, lds_top, hbm_top, best_child, cbranch,
+                                         ordered);
+  lds_flat_fence();
   return n;
 }
 
@@ -1365,7 +1384,8 @@ __device__ __forceinline__ uint32_t reduce_list(const Params& P, const WaveMem& 
                                 uint64_t& best, uint32_t& cbranch, Counters& ctr) {
   const uint32_t lane = lane_id();
   const int K = P.dom_k < kMaxDom ? P.dom_k : kMaxDom;
-  if (K <= 0 || best == ~0ull || len < 2 || len >= (1u << 20)) return len;
+  if (K <= 0 || best == ~0ull || len < 2 || len < (uint32_t)P.reduce_min || len >= (1u << 20))
+    return len;
   const uint64_t tr = P.profile ? cycles() : 0;
   if (len <= (uint32_t)kWave) {
     const uint32_t n =
@@ -1376,6 +1396,7 @@ __device__ __forceinline__ uint32_t reduce_list(const Params& P, const WaveMem& 
     return n;
   }
   const uint32_t minrep = (uint32_t)(best >> 32);
+  lds_flat_fence();  // a list in LDS is reached below through generic pointers
   // dominators: K rounds of a maximum over key = (popcount + 1) << 20 | ~pos
   // among the excluded entries, each round below the previous maximum
   uint32_t dpos[kMaxDom];
@@ -1401,7 +1422,7 @@ __device__ __forceinline__ uint32_t reduce_list(const Params& P, const WaveMem& 
     dpos[nd++] = pd;
     prev = mx;
   }
-  wave_mem_wait();
+  lds_flat_fence();
   if (nd == 0 && !P.dom_skip) {
     if (P.profile) ctr.c_reduce += cycles() - tr;
     return len;
@@ -1460,12 +1481,14 @@ __device__ __forceinline__ uint32_t reduce_list(const Params& P, const WaveMem& 
     ndrop += __popcll(ballot(valid && drop));
     out += __popcll(km);
   }
-  wave_mem_wait();
+  lds_flat_fence();
   best = (wave_min_pair(nb));
   cbranch = nbr;
-  ctr.dropped += ndrop;
-  ctr.skipped += nskip;
-  if (P.profile) ctr.c_reduce += cycles() - tr;
+  if (P.profile) {
+    ctr.dropped += ndrop;
+    ctr.skipped += nskip;
+    ctr.c_reduce += cycles() - tr;
+  }
   return out;
 }
 
@@ -1517,6 +1540,7 @@ __device__ __forceinline__ bool child_node(const Params& P, const WaveMem& m, ui
         else hbm_top = dtop + (uint64_t)clen * sizeof(Rec<1>);
         uint64_t* kLc = (uint64_t*)kaddr;
         Rec<1>* dst = (Rec<1>*)daddr;
+        lds_flat_fence();  // ta.kL and crec are read below through generic pointers
         // keys of the child's L in bit order
         {
           uint32_t t = 0;
@@ -1737,7 +1761,11 @@ __device__ __forceinline__ void dfs(const Params& P, const WaveMem& m, uint32_t 
       else
         o = push_pass<1, false, false, false>(recs, F.len, ax, xidx, crec, prune, ord);
     } else {
+      // generic (flat_) accesses: fenced against the ds_ accesses of LDS lists
+      const bool lds_any = F.tier == 0 || ctier == 0;
+      if (lds_any) lds_flat_fence();
       o = push_pass<W, false>(recs, F.len, ax, xidx, crec, prune, ord);
+      if (lds_any) lds_flat_fence();
     }
     if (P.profile) ctr.c_pass += cycles() - tp;
     ctr.nodes++;
@@ -1860,7 +1888,11 @@ __device__ __forceinline__ void run_items_w(const Params& P, const WaveMem& m, C
     return;
   }
   if (kt == 0) lds0 = (uint32_t)kbytes; else hbm0 = kbytes;
-  for (uint32_t j = lane_id(); j < nL; j += kWave) ((uint64_t*)kaddr)[j] = ld_dev(rta.kL + j);
+  for (uint32_t j = lane_id(); j < nL; j += kWave) {
+    const uint64_t key = ld_dev(rta.kL + j);
+    if (kt == 0) as_lds((uint64_t*)kaddr)[j] = key;
+    else ((uint64_t*)kaddr)[j] = key;
+  }
   wave_mem_wait();
   const TaskArrays ta{(const uint64_t*)kaddr, rta.C};
   for (uint32_t it = 0; it < cnt; ++it) {
@@ -1890,6 +1922,7 @@ __device__ __forceinline__ void run_items_w(const Params& P, const WaveMem& m, C
     }
     uint64_t tp = P.profile ? cycles() : 0;
     PassOut o = push_pass<W, true>(recs, len, ax, xidx, (R_t*)crec_addr, false, P.dom_skip != 0);
+    if (ctier == 0) lds_flat_fence();  // the child list went to LDS through flat_ stores
     if (P.profile) ctr.c_pass += cycles() - tp;
     ctr.nodes++;
     ctr.entries += len;
@@ -2035,7 +2068,7 @@ __device__ __forceinline__ void enumerate_task(const Params& P, const WaveMem& m
   uint32_t lds0 = 0;
   const uint32_t kbytes = (nL * 8 + 15) & ~15u;
   if (kbytes <= m.lds_bytes / 4) {
-    for (uint32_t j = lane; j < nL; j += kWave) ((uint64_t*)m.lds)[j] = m.kL[j];
+    for (uint32_t j = lane; j < nL; j += kWave) as_lds((uint64_t*)m.lds)[j] = m.kL[j];
     kL = (const uint64_t*)m.lds;
     lds0 = kbytes;
   }
@@ -2045,7 +2078,14 @@ __device__ __forceinline__ void enumerate_task(const Params& P, const WaveMem& m
   if ((uint64_t)lds0 + (uint64_t)len * sizeof(R_t) <= m.lds_bytes) {
     R_t* dst = (R_t*)(m.lds + lds0);
     const R_t* src = (const R_t*)list_addr;
-    for (uint32_t i = lane; i < len; i += kWave) dst[i] = src[i];
+    for (uint32_t i = lane; i < len; i += kWave) {
+      auto* q = as_lds(dst) + i;
+      q->minidx = src[i].minidx;
+      q->rep = src[i].rep;
+      q->ksum = src[i].ksum;
+#pragma unroll
+      for (int w = 0; w < Wt; ++w) q->a[w] = src[i].a[w];
+    }
     root = (uint64_t)dst;
     rtier = 0;
     lds_top = lds0 + (uint32_t)((uint64_t)len * sizeof(R_t));

@@ -151,6 +151,14 @@ __device__ __forceinline__ void wave_drain() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
+// A list in LDS may be reached through a generic pointer (flat_ instructions)
+// and through an LDS pointer (ds_ instructions).  The two take different paths
+// to the LDS and are not ordered with each other, so a flat store can land
+// after a later ds_read of the same address (and a ds_write after a later
+// flat load).  Where the access kind of LDS data changes, wait for every
+// outstanding memory operation of the wave.
+__device__ __forceinline__ void lds_flat_fence() { wave_drain(); }
+
 // relaxed device-scope accesses: coherent across the XCDs' L2 caches
 template <class T>
 __device__ __forceinline__ T ld_dev(const T* p) {

@@ -62,7 +62,7 @@ class IterFinderGpu6:
                   "entries_w1", "nodes_w1", "cyc_tail", "sub_pass", "sub_merge",
                   "sub_compress", "sub_publish", "publishes", "merges", "abandoned", "late",
                   "sub_reduce", "dropped", "skipped", "main_kernel_ns", "wide_kernel_ns",
-                  "wide_tasks", "main_nodes", "main_maximal"]
+                  "wide_tasks", "main_nodes", "main_maximal", "main_alg_bytes"]
 
     def stats(self) -> dict:
         out = (C.c_uint64 * 64)()

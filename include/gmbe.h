@@ -256,8 +256,8 @@ int gmbe_finder_exit_times(const gmbe_finder* f, double* out, size_t cap, size_t
  * lists, [24] excluded entries dropped, [25] pushes marked non-maximal;
  * [26] main kernel time and [27] wide-bitset pass time of the last execute
  * (ns, HIP events on the finder's stream), [28] tasks handed to the wide pass,
- * [29] nodes and [30] maximal bicliques of the main kernel alone (the rest
- * are the wide pass's).  The cycle counters ([0..6], [14..18], [23]) and the
+ * [29] nodes, [30] maximal bicliques and [31] algorithmic bytes (with
+ * count_bytes) of the main kernel alone (the rest are the wide pass's).  The cycle counters ([0..6], [14..18], [23]) and the
  * list sizes [8], [12], [13] are filled when the knob "profile" is 1 (else 0);
  * the others always.  Returns the number of counters available. */
 int gmbe_finder_stats(gmbe_finder* f, uint64_t* out, size_t n);
