@@ -278,13 +278,13 @@ struct gmbe_finder {
   int64_t root_begin = 0, root_end = -1;  // processing-order rank range, -1 = all
   int64_t deadline_ms = 0;                 // 0 = run to completion
   int64_t late_share = 32;                 // 0 = off
-  int64_t dom_k = 6, dom_skip = 1;         // list reduction (reduce_list)
+  int64_t dom_k = 8, dom_skip = 1;         // list reduction (reduce_list)
   int64_t recompute = 0;                   // split large tasks without a free context into children
   int64_t count_bytes = 0;                 // count algorithmic bytes (alg_bytes)
   int64_t reduce_min = 0;                  // reduce only lists at least this long
-  int64_t ctx_slots = 8;                   // shared-context slots per wave
-  int64_t profile = 0, group = 1, share = 1, ctx_kb = 4096, share_work = 4096, share_queue = -1;
-  int64_t merge_min = 256, merge_branch = 64;
+  int64_t ctx_slots = 16;                  // shared-context slots per wave
+  int64_t profile = 0, group = 1, share = 1, ctx_kb = 4096, share_work = 1024, share_queue = -1;
+  int64_t merge_min = 256, merge_branch = 16;
   bool arena_set = false, ctx_set = false;  // sizes given by the caller
   unsigned long long prof[mbe::A_COUNT] = {};
   gmbe::Engine eng;

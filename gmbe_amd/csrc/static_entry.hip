@@ -208,13 +208,13 @@ static int execute_static_impl(const gmbe_graph* g, int bound_height, int bound_
   P.group = 1;
   P.share = 1;
   P.merge_min = 256;
-  P.merge_branch = 64;
+  P.merge_branch = 16;
   P.budget = 0;
   P.exit_stamps = nullptr;
   P.late_share = 32;
-  P.share_work = 4096;
+  P.share_work = 1024;
   P.share_queue = (int)nwaves;
-  P.dom_k = 6;
+  P.dom_k = 8;
   P.dom_skip = 1;
   P.recompute = 0;
   P.count_bytes = 0;
