@@ -10,14 +10,18 @@ graph, and the line's count and hash are those of the whole graph (checked
 against the recorded value in profiles/ when one is there).
 
 The graph is resident in HBM before the timed region.  Untimed warmup: the
-first warmup step counts algorithmic bytes (the per-node byte model) on a
-stride-97 sample of all ranks; further ones run a light sample.  The JSON
-line reports maximal bicliques per second over the timed steps, the roofline
-of the enumeration kernel (algorithmic bytes per launch, i.e. the sampled
-bytes per node times the nodes of the launch, over the kernel's average
-launch time from HIP events on the finder's stream; the measured HBM traffic
-per launch from the committed counter summary; the traffic bandwidth beside
-it) and the CPU baseline: the restatement of the reference's CPU finder
+first warmup step counts bytes per node (count_bytes) on the vertices of
+processing rank 0 mod 4K; further ones run a light sample.  The JSON line
+reports maximal bicliques per second over the timed steps, the roofline of
+the dominant kernel (the one with the most time: the wide-bitset pass on this
+graph): its algorithmic bytes per launch -- the bytes of the bitset lists its
+pushes read and write plus the CSR rows its task roots read, per node from
+the counted warmup part, times the launch's nodes -- over its average launch
+time from HIP events on the finder's stream, the measured HBM traffic per
+launch from the committed counter summary, and per kernel the same rate
+under the reference's id-list byte model (4 (deg(x) + |L| + |C|) bytes per
+push), which this kernel's bitset lists undercut by far; and the CPU
+baseline: the restatement of the reference's CPU finder
 (MbeaFinder with MbeaAdvFinder's 2-hop top level) on all of this host's
 hardware threads over a bounded root sample, with the GPU timed on the same
 sample.  cpu_baseline_imbea and cpu_baseline_pmbe time the restatements of
@@ -49,8 +53,10 @@ DIGEST_FILE = os.path.join(ROOT, "profiles", "config2_full_digest.json")
 WORKLOAD = ("MBE Chung-Lu gamma=2.1 |U|=100,000 |V|=200,000 1.5M edges cap 20,000 seed 2; "
             "-t 1 -o 1 -f; bound_height=20 bound_size=1500; whole graph, all 100,000 "
             "top-level vertices, split into K launches by processing rank mod K")
+# the two enumeration kernels of a launch: the main pass and the wide-bitset pass
+KERNELS = {"main": "mbe_persistent<4, 7168, 4, 4>", "wide": "mbe_persistent<4, 8192, 16, 2>"}
 # CPU baseline sample: light top-level vertices the CPU finishes in seconds
-CPU_STRIDE, CPU_FIRST = 499, 4096
+CPU_STRIDE, CPU_FIRST = 499, 8192
 
 
 def cpu_model() -> str:
@@ -83,16 +89,17 @@ def cpu_baseline(graph, seconds: float, algo: str = "mbea"):
     t0 = time.perf_counter()
     r = run(og, threads=threads, stride=CPU_STRIDE, phase=CPU_FIRST, time_limit=seconds)
     wall = time.perf_counter() - t0
-    complete = r.tasks >= n_roots
+    complete = r.tasks >= n_roots and r.cut == 0
     order = (f"; ranks in the containment order, built in {wall - r.seconds:.1f} s before the "
              "search" if algo == "pmbe" else "")
     return {
         "value": r.maximal / r.seconds if r.seconds > 0 else 0.0, "unit": "bicliques/s",
         "cores": threads, "kind": "port", "cpu": cpu_model(),
         "sample": (f"{name}, {threads} threads) over top-level vertices of processing rank "
-                   f"{CPU_FIRST} + {CPU_STRIDE} t ({n_roots} vertices{order}): {r.tasks} done, "
-                   f"{r.maximal} maximal bicliques in {r.seconds:.1f} s"
-                   + ("" if complete else " (cut by the time limit)")),
+                   f"{CPU_FIRST} + {CPU_STRIDE} t ({n_roots} vertices{order}): "
+                   + (f"all done, {r.maximal} maximal bicliques in {r.seconds:.1f} s" if complete
+                      else f"cut by the time limit after {r.seconds:.1f} s with {r.maximal} "
+                           f"maximal bicliques ({r.tasks} vertices started)")),
         "complete": bool(complete), "digest": [int(r.maximal), int(r.hash_sum), int(r.hash_xor)],
     }
 
@@ -122,9 +129,9 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
 
 
-def measured_traffic():
-    """HBM bytes per launch of the enumeration kernel from the committed
-    counter summary, when it was taken on this workload."""
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of one enumeration kernel ("main" or "wide") from
+    the committed counter summary, when it was taken on this workload."""
     try:
         with open(TRAFFIC_FILE) as fh:
             t = json.load(fh)
@@ -132,7 +139,7 @@ def measured_traffic():
         return None
     if t.get("workload") != WORKLOAD:
         return None
-    return t.get("hbm_bytes_per_launch")
+    return t.get("hbm_bytes_per_launch", {}).get(kernel)
 
 
 def bench_batch(args):
@@ -369,7 +376,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--only", type=int, default=0, help="time only the first N of the K parts")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--workload", choices=["plain", "batch", "serve"], default="plain")
@@ -407,16 +414,31 @@ def main():
           f"(built in {time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
     K = max(1, args.steps)
     f = IterFinderGpu6(graph, workloads.BOUND_HEIGHT, workloads.BOUND_SIZE)
-    # warmup: byte model on a stride-97 sample of all ranks, then light samples
-    bytes_per_node = None
+    # warmup: the byte model of each kernel on a 1/(4K) part of the graph
+    # (processing ranks 0 mod 4K, counted with count_bytes), then light samples
+    model = None
     for i in range(args.warmup):
-        f.set("root_stride", 97)
-        f.set("root_phase", 0 if i == 0 else 1024 % 97)
-        f.set("root_begin", 0 if i == 0 else 1024)
-        f.set("count_bytes", 1 if i == 0 else 0)
+        if i == 0:
+            f.set("root_stride", 4 * K)
+            f.set("root_phase", 0)
+            f.set("root_begin", 0)
+            f.set("count_bytes", 1)
+        else:
+            f.set("root_stride", 97)
+            f.set("root_phase", 1024 % 97)
+            f.set("root_begin", 1024)
+            f.set("count_bytes", 0)
         r = f.execute()
         if i == 0:
-            bytes_per_node = r.alg_bytes / max(1, r.nodes)
+            sw = f.stats()
+            mn, mb, ml = sw["main_nodes"], sw["main_alg_bytes"], sw["main_list_bytes"]
+            wn = max(1, r.nodes - mn)
+            model = {"main": {"list": ml / max(1, mn), "survey": mb / max(1, mn)},
+                     "wide": {"list": (sw["list_bytes"] - ml) / wn,
+                              "survey": (r.alg_bytes - mb) / wn},
+                     "sample": f"processing ranks 0 mod {4 * K}: {r.nodes} nodes, "
+                               f"{sw['list_bytes']} list bytes, {r.alg_bytes} bytes of the "
+                               "id-list model"}
         print(f"# warmup {i}: {r.maximal} bicliques in {r.exe_time_s:.3f} s", file=sys.stderr,
               flush=True)
     f.set("count_bytes", 0)
@@ -429,21 +451,20 @@ def main():
         f.set("root_phase", i)
         r = f.execute()  # synchronous: ends with the counts on the host
         sts = f.stats()
-        steps.append((r.maximal, r.hash_sum, r.hash_xor, r.nodes, r.kernel_time_s,
-                      sts["main_kernel_ns"] * 1e-9, sts["wide_kernel_ns"] * 1e-9, r.exe_time_s))
+        steps.append({"digest": r.digest(), "nodes": r.nodes, "main_nodes": sts["main_nodes"],
+                      "main_s": sts["main_kernel_ns"] * 1e-9,
+                      "wide_s": sts["wide_kernel_ns"] * 1e-9, "exe_s": r.exe_time_s})
         print(f"# step {i}: {r.maximal} bicliques, exe {r.exe_time_s:.3f} s, main kernel "
               f"{sts['main_kernel_ns'] * 1e-9:.3f} s, wide pass {sts['wide_kernel_ns'] * 1e-9:.3f} s",
               file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t_start
     f.close()
-    total = sum(x[0] for x in steps)
-    hsum = sum(x[1] for x in steps) % (1 << 64)
+    total = sum(x["digest"][0] for x in steps)
+    hsum = sum(x["digest"][1] for x in steps) % (1 << 64)
     hxor = 0
     for x in steps:
-        hxor ^= x[2]
-    nodes = sum(x[3] for x in steps)
-    main_avg = sum(x[5] for x in steps) / len(steps)
-    nodes_avg = nodes / len(steps)
+        hxor ^= x["digest"][2]
+    nodes = sum(x["nodes"] for x in steps)
     recorded = None
     if parts == K:
         try:
@@ -453,8 +474,30 @@ def main():
                 [total, hsum, hxor]
         except (OSError, ValueError, KeyError):
             recorded = None
-    achieved = bytes_per_node * nodes_avg / main_avg if bytes_per_node and main_avg > 0 else 0.0
-    traffic = measured_traffic()
+    # per kernel: average launch time (HIP events) and nodes per launch; the
+    # dominant kernel (most time) carries the roofline
+    kern = {
+        "main": {"s": sum(x["main_s"] for x in steps),
+                 "nodes": sum(x["main_nodes"] for x in steps), "launches": len(steps)},
+        "wide": {"s": sum(x["wide_s"] for x in steps),
+                 "nodes": sum(x["nodes"] - x["main_nodes"] for x in steps),
+                 "launches": sum(1 for x in steps if x["wide_s"] > 0)},
+    }
+    for name, kd in kern.items():
+        n = max(1, kd["launches"])
+        kd["avg_launch_s"] = kd["s"] / n
+        kd["nodes_per_launch"] = kd["nodes"] / n
+        for b in ("list", "survey"):
+            bpn = model[name][b] if model else None
+            per_launch = bpn * kd["nodes_per_launch"] if bpn is not None else None
+            kd[f"{b}_bytes_per_node"] = bpn
+            kd[f"{b}_bytes_per_launch"] = per_launch
+            kd[f"{b}_GBs"] = (per_launch / kd["avg_launch_s"] / 1e9
+                              if per_launch is not None and kd["avg_launch_s"] > 0 else None)
+    dom = max(kern, key=lambda k: kern[k]["s"])
+    kd = kern[dom]
+    achieved = kd["list_GBs"] * 1e9 if kd["list_GBs"] else 0.0
+    traffic = measured_traffic(dom)
     line = {
         "metric": METRIC,
         "value": total / elapsed,
@@ -474,17 +517,19 @@ def main():
             "parts": K, "parts_timed": parts,
             "maximal_bicliques": int(total), "hash_sum": f"{hsum:#018x}",
             "hash_xor": f"{hxor:#018x}", "matches_recorded_digest": recorded,
-            "nodes": int(nodes), "main_kernel_s_per_launch": main_avg,
-            "wide_pass_s_total": sum(x[6] for x in steps),
-            "exe_time_s_total": sum(x[7] for x in steps),
+            "nodes": int(nodes), "nodes_per_s": nodes / elapsed,
+            "exe_time_s_total": sum(x["exe_s"] for x in steps),
+            "kernels": kern,
         },
         "roofline": {
-            "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK,
+            "bound": "hbm", "kernel": KERNELS[dom], "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
             "traffic": traffic,
-            "traffic_bandwidth_GBs": (traffic / main_avg / 1e9) if traffic and main_avg else None,
-            "traffic_frac": (traffic / main_avg / HBM_PEAK) if traffic and main_avg else None,
-            "alg_bytes_per_node": bytes_per_node,
+            "traffic_bandwidth_GBs": (traffic / kd["avg_launch_s"] / 1e9)
+            if traffic and kd["avg_launch_s"] else None,
+            "traffic_frac": (traffic / kd["avg_launch_s"] / HBM_PEAK)
+            if traffic and kd["avg_launch_s"] else None,
+            "byte_model": model,
         },
     }
     if not args.no_cpu:

@@ -718,10 +718,10 @@ int gmbe_finder_list(gmbe_finder* f, uint32_t* buf, size_t cap, size_t* used) {
 static int gmbe_finder_stats_impl(gmbe_finder* f, uint64_t* out, size_t n) {
   if (!f) return set_error(GMBE_ERR_ARG, "null finder");
   // the profile counters, then the main kernel's and the wide pass's time (ns),
-  // the tasks handed to the wide pass, and the nodes, maximal bicliques and
-  // algorithmic bytes of the main kernel alone
+  // the tasks handed to the wide pass, and the nodes, maximal bicliques,
+  // algorithmic bytes and list bytes of the main kernel alone
   const int first = mbe::A_CYC_WAIT, nprof = mbe::A_COUNT - mbe::A_CYC_WAIT;
-  uint64_t all[mbe::A_COUNT + 6];
+  uint64_t all[mbe::A_COUNT + 7];
   for (int i = 0; i < nprof; ++i) all[i] = f->prof[first + i];
   all[nprof] = (uint64_t)(f->main_ms * 1e6);
   all[nprof + 1] = (uint64_t)(f->wide_ms * 1e6);
@@ -729,7 +729,8 @@ static int gmbe_finder_stats_impl(gmbe_finder* f, uint64_t* out, size_t n) {
   all[nprof + 3] = f->main_acc[mbe::A_NODES];
   all[nprof + 4] = f->main_acc[mbe::A_MAXIMAL];
   all[nprof + 5] = f->main_acc[mbe::A_BALG];
-  const int avail = nprof + 6;
+  all[nprof + 6] = f->main_acc[mbe::A_BLIST];
+  const int avail = nprof + 7;
   for (size_t i = 0; out && i < n && i < (size_t)avail; ++i) out[i] = all[i];
   return avail;
 }

@@ -87,7 +87,9 @@ enum Acc : int {
   A_CYC_WAIT, A_CYC_PREFIX, A_CYC_EXPAND, A_CYC_EXTRACT, A_CYC_BUILD, A_CYC_DFS, A_CYC_SPLIT,
   A_EDGES, A_ENTRIES, A_OVF_ARENA, A_OVF_WIDE, A_OVF_DEFER, A_ENT_W1, A_NODES_W1, A_CYC_TAIL,
   A_CYC_PASS, A_CYC_MERGE, A_CYC_COMPRESS, A_CYC_PUBLISH, A_PUBLISHES, A_MERGES, A_ABANDONED,
-  A_LATE, A_CYC_REDUCE, A_DROPPED, A_SKIPPED, A_COUNT
+  A_LATE, A_CYC_REDUCE, A_DROPPED, A_SKIPPED,
+  A_BLIST,  // with count_bytes: bytes of the lists pushes read and write, CSR rows of task roots
+  A_COUNT
 };
 
 struct Sched {
@@ -189,6 +191,7 @@ struct Counters {
   LdsCount abandoned;         // subtrees left unfinished at the deadline
   LdsCount late;              // frames handed over by late sharing
   LdsCount c_reduce, dropped, skipped;  // list reduction: cycles, entries
+  LdsCount blist;  // bytes of the bitset lists pushes read and write, CSR bytes of task roots
 };
 constexpr int kCounters = (int)(sizeof(Counters) / sizeof(LdsCount));
 
@@ -1769,6 +1772,7 @@ __device__ __forceinline__ void dfs(const Params& P, const WaveMem& m, uint32_t 
     }
     if (P.profile) ctr.c_pass += cycles() - tp;
     ctr.nodes++;
+    if (P.count_bytes) ctr.blist += (uint64_t)(F.len + o.clen) * sizeof(R_t);
     if (P.profile) {
       ctr.entries += F.len;
       if (W == 1) {
@@ -1925,6 +1929,7 @@ __device__ __forceinline__ void run_items_w(const Params& P, const WaveMem& m, C
     if (ctier == 0) lds_flat_fence();  // the child list went to LDS through flat_ stores
     if (P.profile) ctr.c_pass += cycles() - tp;
     ctr.nodes++;
+    if (P.count_bytes) ctr.blist += (uint64_t)(len + o.clen) * sizeof(R_t);
     ctr.entries += len;
     if (W == 1) {
       ctr.ent_w1 += len;
@@ -2010,6 +2015,7 @@ __device__ __forceinline__ void enumerate_task(const Params& P, const WaveMem& m
   }
   const uint64_t braw = (uint64_t)m.arena;
   hbm_top = (uint64_t)nC * sizeof(R_t);
+  if (P.count_bytes) ctr.blist += (uint64_t)nC * sizeof(R_t);
   R_t* brec = (R_t*)braw;
   for (uint32_t i = lane; i < nC; i += kWave) {
     brec[i].minidx = i;
@@ -2173,7 +2179,10 @@ __device__ __forceinline__ void run_task(const Params& P, const WaveMem& m, cons
     L = m.L;
   }
   if (nL == 0) {  // isolated vertex: no biclique (undefined in the reference, Q5)
-    if (P.count_bytes) ctr.balg += 4ull * sdeg;
+    if (P.count_bytes) {
+      ctr.balg += 4ull * sdeg;
+      ctr.blist += 4ull * sdeg;
+    }
     ctr.nonmax++;
     return;
   }
@@ -2191,7 +2200,10 @@ __device__ __forceinline__ void run_task(const Params& P, const WaveMem& m, cons
       if (o0 == 0) or_wg(bm1 + (w >> 12), 1ull << ((w >> 6) & 63));
     }
   });
-  if (P.count_bytes) ctr.balg += 4ull * (sdeg + root_edges);
+  if (P.count_bytes) {
+    ctr.balg += 4ull * (sdeg + root_edges);
+    ctr.blist += 4ull * (sdeg + root_edges);
+  }
   ctr.edges += root_edges;
   wave_mem_wait();
   phase(ctr, P.profile, 2, t0);

@@ -97,7 +97,7 @@ void mbe_persistent(Params P) {
       {A_CYC_PASS, &c.c_pass}, {A_CYC_MERGE, &c.c_merge}, {A_CYC_COMPRESS, &c.c_compress},
       {A_CYC_PUBLISH, &c.c_publish}, {A_PUBLISHES, &c.publishes}, {A_MERGES, &c.merges},
       {A_ABANDONED, &c.abandoned}, {A_LATE, &c.late}, {A_CYC_REDUCE, &c.c_reduce},
-      {A_DROPPED, &c.dropped}, {A_SKIPPED, &c.skipped}};
+      {A_DROPPED, &c.dropped}, {A_SKIPPED, &c.skipped}, {A_BLIST, &c.blist}};
   for (const Slot& sl : slots) {
     const uint64_t v = sl.v->get();
     if (lane_id() == 0 && v != 0)

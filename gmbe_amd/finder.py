@@ -61,8 +61,9 @@ class IterFinderGpu6:
                   "cyc_split", "edges", "entries", "ovf_arena", "ovf_wide", "ovf_defer",
                   "entries_w1", "nodes_w1", "cyc_tail", "sub_pass", "sub_merge",
                   "sub_compress", "sub_publish", "publishes", "merges", "abandoned", "late",
-                  "sub_reduce", "dropped", "skipped", "main_kernel_ns", "wide_kernel_ns",
-                  "wide_tasks", "main_nodes", "main_maximal", "main_alg_bytes"]
+                  "sub_reduce", "dropped", "skipped", "list_bytes", "main_kernel_ns",
+                  "wide_kernel_ns", "wide_tasks", "main_nodes", "main_maximal",
+                  "main_alg_bytes", "main_list_bytes"]
 
     def stats(self) -> dict:
         out = (C.c_uint64 * 64)()

@@ -254,12 +254,15 @@ int gmbe_finder_exit_times(const gmbe_finder* f, double* out, size_t cap, size_t
  * published, [20] twin merges; [21] subtrees abandoned at the deadline;
  * [22] frames handed over by late sharing; [23] cycles spent reducing
  * lists, [24] excluded entries dropped, [25] pushes marked non-maximal;
- * [26] main kernel time and [27] wide-bitset pass time of the last execute
- * (ns, HIP events on the finder's stream), [28] tasks handed to the wide pass,
- * [29] nodes, [30] maximal bicliques and [31] algorithmic bytes (with
- * count_bytes) of the main kernel alone (the rest are the wide pass's).  The cycle counters ([0..6], [14..18], [23]) and the
- * list sizes [8], [12], [13] are filled when the knob "profile" is 1 (else 0);
- * the others always.  Returns the number of counters available. */
+ * [26] bytes of the bitset lists that pushes read and write plus the CSR
+ * bytes of task roots (with the knob "count_bytes");
+ * [27] main kernel time and [28] wide-bitset pass time of the last execute
+ * (ns, HIP events on the finder's stream), [29] tasks handed to the wide
+ * pass; [30] nodes, [31] maximal bicliques, [32] algorithmic bytes and
+ * [33] list bytes (both with count_bytes) of the main kernel alone (the rest
+ * are the wide pass's).  The cycle counters ([0..6], [14..18], [23]) and
+ * [8], [12], [13], [24], [25] are filled when the knob "profile" is 1
+ * (else 0); the others always.  Returns the number of counters available. */
 int gmbe_finder_stats(gmbe_finder* f, uint64_t* out, size_t n);
 
 /* Replaces the finder destructor chain (src/IterFinderGpu.cu:1087-1093). */
