@@ -282,6 +282,7 @@ struct gmbe_finder {
   int64_t recompute = 0;                   // split large tasks without a free context into children
   int64_t count_bytes = 0;                 // count algorithmic bytes (alg_bytes)
   int64_t reduce_min = 0;                  // reduce only lists at least this long
+  int64_t wide_resident = 1;               // wide pass: launch only the blocks that fit at once
   int64_t ctx_slots = 16;                  // shared-context slots per wave
   int64_t profile = 0, group = 1, share = 1, ctx_kb = 4096, share_work = 1024, share_queue = -1;
   int64_t merge_min = 256, merge_branch = 16;
@@ -469,6 +470,7 @@ static int gmbe_finder_set_impl(gmbe_finder* f, const char* key, int64_t value) 
   else if (k == "recompute") { if ((rc = range(0, 1))) return rc; f->recompute = value; }
   else if (k == "count_bytes") { if ((rc = range(0, 1))) return rc; f->count_bytes = value; }
   else if (k == "reduce_min") { if ((rc = range(0, 1 << 20))) return rc; f->reduce_min = value; }
+  else if (k == "wide_resident") { if ((rc = range(0, 1))) return rc; f->wide_resident = value; }
   else if (k == "deadline_ms") { if ((rc = range(0, 1ll << 32))) return rc; f->deadline_ms = value; }
   else if (k == "root_begin") { if ((rc = range(0, (1ll << 32)))) return rc; f->root_begin = value; }
   else if (k == "root_end") { if ((rc = range(-1, (1ll << 32)))) return rc; f->root_end = value; }
@@ -598,7 +600,12 @@ static int gmbe_finder_execute_impl(gmbe_finder* f, gmbe_result* res) {
     P.s.deferred = nullptr;
     P.s.deferred_cap = 0;
     KernelChoice wk = pick_kernel(8, 4, true);
-    hipLaunchKernelGGL(wk.fn, dim3(f->nblocks), dim3(wk.waves * 64), 0, f->stream, P);
+    // the wide kernel holds fewer waves per SIMD than the main one: launch the
+    // blocks that are resident at once (every wave index stays within the slabs)
+    const int wblocks = f->wide_resident
+                            ? std::min(f->nblocks, resident_blocks(wk, f->eng.cu_count))
+                            : f->nblocks;
+    hipLaunchKernelGGL(wk.fn, dim3(wblocks), dim3(wk.waves * 64), 0, f->stream, P);
     HIP_TRY(hipGetLastError());
   }
   HIP_TRY(hipEventRecord(f->ev1, f->stream));
