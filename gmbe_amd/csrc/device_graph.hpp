@@ -34,6 +34,7 @@ mbe::Scratch scratch_layout(uint32_t nV, uint32_t dmaxV, uint64_t arena_bytes,
 // Control counters live kCtlStride u64 (256 bytes) apart so that polling one
 // never contends with atomics on another; the shards' head/tail pairs follow.
 enum CtlSlot { C_ROOT_NEXT = 0, C_WORK_LEFT, C_QUEUED, C_LIST_CURSOR, C_DEFERRED, C_WIDE_N,
+               C_DEADLINE,  // absolute deadline of the run (wall clock), shared by both passes
                C_QCTL = 8, C_NSLOTS = 8 + 2 * mbe::kShards };
 constexpr uint64_t kCtlWords = (uint64_t)C_NSLOTS * mbe::kCtlStride;
 
@@ -71,7 +72,8 @@ hipError_t launch_fill(hipStream_t st, unsigned long long* p, unsigned long long
 // Control block for a new launch: zero, then work_left = n_roots.
 hipError_t launch_ctl_start(hipStream_t st, unsigned long long* ctl, unsigned long long n_roots);
 // Control block for the wide pass over the tasks the main pass deferred:
-// C_WIDE_N = min(n_deferred, cap), work_left = C_WIDE_N, the rest zero.
+// C_WIDE_N = min(n_deferred, cap), work_left = C_WIDE_N, the list cursor and
+// the deadline kept, the rest zero.
 hipError_t launch_ctl_wide(hipStream_t st, unsigned long long* ctl, unsigned long long cap);
 
 }  // namespace gmbe

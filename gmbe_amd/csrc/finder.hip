@@ -61,14 +61,15 @@ __global__ void ctl_start(unsigned long long* ctl, unsigned long long words, uns
 // deferred tasks stay unfinished and their graphs incomplete.
 __global__ void ctl_wide(unsigned long long* ctl, unsigned long long words, unsigned long long wl,
                          unsigned long long nd, unsigned long long wn, unsigned long long cur,
-                         unsigned long long cap) {
+                         unsigned long long dlw, unsigned long long cap) {
   const unsigned long long n = ctl[wl] != 0 ? 0ull : ctl[nd] < cap ? ctl[nd] : cap;
-  const unsigned long long cursor = ctl[cur];
+  const unsigned long long cursor = ctl[cur], deadline = ctl[dlw];
   __syncthreads();
   for (unsigned long long i = threadIdx.x; i < words; i += blockDim.x) {
     unsigned long long v = 0ull;
     if (i == wl || i == wn) v = n;
     if (i == cur) v = cursor;
+    if (i == dlw) v = deadline;  // the main pass's deadline holds for the wide pass
     ctl[i] = v;
   }
 }
@@ -148,7 +149,8 @@ hipError_t launch_ctl_wide(hipStream_t st, unsigned long long* ctl, unsigned lon
                      (unsigned long long)C_WORK_LEFT * kCtlStride,
                      (unsigned long long)C_DEFERRED * kCtlStride,
                      (unsigned long long)C_WIDE_N * kCtlStride,
-                     (unsigned long long)C_LIST_CURSOR * kCtlStride, cap);
+                     (unsigned long long)C_LIST_CURSOR * kCtlStride,
+                     (unsigned long long)C_DEADLINE * kCtlStride, cap);
   return hipGetLastError();
 }
 
@@ -525,6 +527,7 @@ static int gmbe_finder_execute_impl(gmbe_finder* f, gmbe_result* res) {
   P.s.queued = f->ctl + C_QUEUED * kCtlStride;
   P.list_cursor = f->ctl + C_LIST_CURSOR * kCtlStride;
   P.s.n_deferred = f->ctl + C_DEFERRED * kCtlStride;
+  P.s.deadline_word = f->ctl + C_DEADLINE * kCtlStride;
   P.s.deferred = f->deferred;
   P.s.deferred_cap = f->deferred_cap;
   P.s.q = f->q;
@@ -590,6 +593,7 @@ static int gmbe_finder_execute_impl(gmbe_finder* f, gmbe_result* res) {
     std::vector<unsigned long long> ctl2(C_NSLOTS * kCtlStride, 0ull);
     ctl2[C_WORK_LEFT * kCtlStride] = f->last_deferred;
     ctl2[C_LIST_CURSOR * kCtlStride] = ctl_out[C_LIST_CURSOR * kCtlStride];
+    ctl2[C_DEADLINE * kCtlStride] = ctl_out[C_DEADLINE * kCtlStride];
     HIP_TRY(hipMemcpyAsync(f->ctl, ctl2.data(), ctl2.size() * 8, hipMemcpyHostToDevice,
                            f->stream));
     hipLaunchKernelGGL(mbe::init_queue, dim3(256), dim3(256), 0, f->stream, f->qseq,

@@ -334,6 +334,7 @@ mbe::Params Lane::params(bool wide) const {
   P.s.queued = ctl + C_QUEUED * kCtlStride;
   P.list_cursor = ctl + C_LIST_CURSOR * kCtlStride;
   P.s.n_deferred = ctl + C_DEFERRED * kCtlStride;
+  P.s.deadline_word = ctl + C_DEADLINE * kCtlStride;  // one absolute deadline for both passes
   P.s.q = q;
   P.s.qseq = qseq;
   P.s.qcap = qcap;

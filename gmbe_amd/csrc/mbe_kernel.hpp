@@ -102,6 +102,7 @@ struct Sched {
   unsigned long long qcap;        // slots per shard, power of two
   unsigned long long n_roots;     // top-level vertices of this run (sample)
   const unsigned long long* n_roots_dev;  // if set, the count is read here instead
+  unsigned long long* deadline_word;      // if set: the run's absolute deadline (0 = not yet armed)
   const unsigned long long* root_base;    // batch: first root of the j-th graph in order
   const uint32_t* order;                  // batch: graph id of the j-th graph in order
   uint32_t n_graphs;

@@ -24,7 +24,23 @@ void mbe_persistent(Params P) {
   const uint64_t gw = (uint64_t)blockIdx.x * WAVES + wv;
   WaveMem m = wave_mem(P, gw, lds + wv * (LDS_PER_WAVE > 0 ? LDS_PER_WAVE : 16), LDS_PER_WAVE,
                        s_off[wv], s_start[wv], s_fr[wv]);
-  m.deadline = P.budget != 0 ? wall_clock64() + P.budget : 0;
+  // the deadline: budget ticks after the first wave of the run started; with a
+  // deadline word every later wave and the wide pass read the same absolute
+  // time instead of arming their own
+  uint64_t dl = 0;
+  if (P.budget != 0) {
+    dl = wall_clock64() + P.budget;
+    if (P.s.deadline_word) {
+      unsigned long long cur = 0;
+      if (lane_id() == 0 &&
+          !__hip_atomic_compare_exchange_strong(P.s.deadline_word, &cur, (unsigned long long)dl,
+                                                __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT))
+        dl = cur;
+      dl = uni64(dl);
+    }
+  }
+  m.deadline = dl;
   __shared__ Counters s_ctr[WAVES];
   Counters& c = s_ctr[wv];
   for (int i = (int)lane_id(); i < kCounters; i += kWave) as_lds(&c.maximal)[i].v = 0;
