@@ -162,12 +162,15 @@ def bench_batch(args):
     if not args.no_autotune:
         b0.set("verbose", 1)
         b0.set("autotune", 1)
-    choice = {"waves_per_block": b0.get("waves_per_block"), "lds_kb": b0.get("lds_kb"),
+    ncls = b0.get("n_classes")
+    choice = {"variant_by_class": [[b0.get(f"waves_per_block.{c}"), b0.get(f"lds_kb.{c}")]
+                                   for c in range(ncls)],
               "bounds_by_class": [[b0.get(f"bound_height.{c}"), b0.get(f"bound_size.{c}")]
-                                  for c in range(b0.get("n_classes"))]}
+                                  for c in range(ncls)]}
     for b in batches[1:]:
-        b.set("waves_per_block", choice["waves_per_block"])
-        b.set("lds_kb", choice["lds_kb"])
+        for c, (w, l) in enumerate(choice["variant_by_class"]):
+            b.set(f"waves_per_block.{c}", w)
+            b.set(f"lds_kb.{c}", l)
         for c, (bh, bs) in enumerate(choice["bounds_by_class"]):
             b.set(f"bound_height.{c}", bh)
             b.set(f"bound_size.{c}", bs)

@@ -1,7 +1,7 @@
-// batch.hip — batched form: many small graphs in one persistent launch,
-// captured in a HIP graph and replayed, with an autotuner that picks the
-// kernel variant (waves per block, LDS list arena per wave) and the split
-// thresholds of each graph size class.  No reference counterpart: the
+// batch.hip — batched form: many small graphs in one captured run (one
+// persistent launch pair per graph size class), replayed from a HIP graph,
+// with an autotuner that picks each size class's kernel variant (waves per
+// block, LDS list arena per wave) and split thresholds.  No reference counterpart: the
 // reference runs one graph per process (src/main.cu:126-147).
 #include <hip/hip_runtime.h>
 
@@ -19,7 +19,7 @@ using namespace gmbe;
 
 namespace {
 
-constexpr int kClasses = 4;
+constexpr int kClasses = kLaneClasses;
 // size classes by edge count: [0, 2k), [2k, 8k), [8k, 32k), [32k, inf)
 int size_class(uint64_t E) { return E < 2000 ? 0 : E < 8000 ? 1 : E < 32000 ? 2 : 3; }
 
@@ -57,16 +57,19 @@ void drop_capture(gmbe_batch* b) {
   b->captured = false;
 }
 
-// Load the lane with the graphs `subset` (all graphs when empty).
+// Load the lane with the graphs `subset` (all graphs when empty), in one
+// claim segment per size class.
 int repack(gmbe_batch* b, const std::vector<size_t>& subset = {}) {
   b->pack.clear();
   const size_t n = subset.empty() ? b->graphs.size() : subset.size();
+  std::vector<int> cls(n);
   for (size_t j = 0; j < n; ++j) {
     const size_t i = subset.empty() ? j : subset[j];
     const int c = b->cls[i];
     b->pack.add(*b->graphs[i], b->cls_bh[c], b->cls_bs[c]);
+    cls[j] = c;
   }
-  b->pack.finish();
+  b->pack.finish_classes(cls, kClasses);
   int rc = b->lane.upload(b->pack);
   if (rc) return rc;
   if (hipStreamSynchronize(b->lane.st) != hipSuccess)
@@ -130,7 +133,7 @@ int run_looped(gmbe_batch* b, std::vector<gmbe_result>& out) {
     one.clear();
     const int c = b->cls[i];
     one.add(*b->graphs[i], b->cls_bh[c], b->cls_bs[c]);
-    one.finish();
+    one.finish_classes({c}, kClasses);  // the graph's class kernel
     int rc = b->lane.upload(one);
     if (rc) return rc;
     if (hipEventRecord(b->lane.ev0, b->lane.st) != hipSuccess)
@@ -152,14 +155,27 @@ int run_looped(gmbe_batch* b, std::vector<gmbe_result>& out) {
   return GMBE_OK;
 }
 
-int set_variant(gmbe_batch* b, int waves, int lds_kb) {
+// The kernel variant of every class (c < 0) or of class c.
+int set_variant(gmbe_batch* b, int waves, int lds_kb, int c = -1) {
   if (!pick_kernel(lds_kb, waves, false).fn)
     return set_error(GMBE_ERR_ARG, "no kernel with %d waves per block and lds_kb=%d", waves,
                      lds_kb);
-  b->lane.knobs.waves = waves;
-  b->lane.knobs.lds_kb = lds_kb;
+  if (c < 0) {
+    b->lane.knobs.waves = waves;
+    b->lane.knobs.lds_kb = lds_kb;
+    for (int k = 0; k < kClasses; ++k) b->lane.cls_waves[k] = b->lane.cls_lds[k] = -1;
+  } else {
+    b->lane.cls_waves[c] = waves;
+    b->lane.cls_lds[c] = lds_kb;
+  }
   drop_capture(b);
   return b->lane.alloc_run();
+}
+int class_waves(const gmbe_batch* b, int c) {
+  return b->lane.cls_waves[c] > 0 ? b->lane.cls_waves[c] : b->lane.knobs.waves;
+}
+int class_lds(const gmbe_batch* b, int c) {
+  return b->lane.cls_lds[c] >= 0 ? b->lane.cls_lds[c] : b->lane.knobs.lds_kb;
 }
 
 // Time one trial run (median of `reps`), recorded eagerly under a deadline:
@@ -188,49 +204,43 @@ int timed(gmbe_batch* b, const std::vector<size_t>& sample, int reps, double bud
   return GMBE_OK;
 }
 
-// Autotuner: the kernel variant for the whole batch, then each size class's
-// split thresholds with the others fixed, each by the measured wall time of a
-// stratified sample of the batch (best of two runs; slow candidates are cut
-// short by the kernel's deadline).  The captured launch is rebuilt afterwards.
+// Autotuner: for each size class, its kernel variant and then its split
+// thresholds, each by the measured wall time of a sample of the class's
+// graphs (best of two runs; slow candidates are cut short by the kernel's
+// deadline).  The captured run is rebuilt afterwards.
 int autotune(gmbe_batch* b) {
-  // a stratified sample: up to kSample graphs of each size class, spread
-  // over the class's members
+  // per class: up to kSample graphs spread over the class's members
   constexpr size_t kSample = 8;
-  std::vector<size_t> sample;
-  for (int c = 0; c < kClasses; ++c) {
-    std::vector<size_t> mem;
-    for (size_t i = 0; i < b->cls.size(); ++i)
-      if (b->cls[i] == c) mem.push_back(i);
-    const size_t step = std::max<size_t>(1, mem.size() / kSample);
-    for (size_t j = 0; j < mem.size() && j / step < kSample; j += step) sample.push_back(mem[j]);
-  }
-  double best = 1e300;
-  int bw = b->lane.knobs.waves, bl = b->lane.knobs.lds_kb;
   // trials are cut at twice the best time so far (at least 200 ms)
   auto budget = [&](double cur) { return std::max(0.2, 2.0 * std::min(cur, 1e3)); };
-  for (const auto& v : kVariants) {
-    int rc = set_variant(b, v[0], v[1]);
-    if (rc) return rc;
-    double s = 0;
-    rc = timed(b, sample, 2, budget(best), &s);
-    if (rc) return rc;
-    if (b->verbose) {
-      fprintf(stderr, "autotune: %zu graphs, waves_per_block=%d lds_kb=%d: %.4f s\n",
-              sample.size(), v[0], v[1], s);
-      fflush(stderr);
-    }
-    if (s < best) {
-      best = s;
-      bw = v[0];
-      bl = v[1];
-    }
-  }
-  int rc = set_variant(b, bw, bl);
-  if (rc) return rc;
   for (int c = 0; c < kClasses; ++c) {
-    bool present = false;
-    for (int k : b->cls) present = present || k == c;
-    if (!present) continue;
+    std::vector<size_t> mem, sample;
+    for (size_t i = 0; i < b->cls.size(); ++i)
+      if (b->cls[i] == c) mem.push_back(i);
+    if (mem.empty()) continue;
+    const size_t step = std::max<size_t>(1, mem.size() / kSample);
+    for (size_t j = 0; j < mem.size() && j / step < kSample; j += step) sample.push_back(mem[j]);
+    double best = 1e300;
+    int bw = class_waves(b, c), bl = class_lds(b, c);
+    for (const auto& v : kVariants) {
+      int rc = set_variant(b, v[0], v[1], c);
+      if (rc) return rc;
+      double s = 0;
+      rc = timed(b, sample, 2, budget(best), &s);
+      if (rc) return rc;
+      if (b->verbose) {
+        fprintf(stderr, "autotune: class %d, %zu graphs, waves_per_block=%d lds_kb=%d: %.4f s\n",
+                c, sample.size(), v[0], v[1], s);
+        fflush(stderr);
+      }
+      if (s < best) {
+        best = s;
+        bw = v[0];
+        bl = v[1];
+      }
+    }
+    int rc = set_variant(b, bw, bl, c);
+    if (rc) return rc;
     int bh = b->cls_bh[c], bs = b->cls_bs[c];
     double cbest = 1e300;
     for (const auto& t : kBounds) {
@@ -341,16 +351,23 @@ static int gmbe_batch_set_impl(gmbe_batch* b, const char* key, int64_t value) {
     b->per_graph_launch = value;
     return GMBE_OK;
   }
-  if (k == "waves_per_block") {
+  if (k == "waves_per_block" || k.rfind("waves_per_block.", 0) == 0) {
+    // every class, or class c with "waves_per_block.c"
+    const int c = k.size() > 15 ? atoi(k.c_str() + 16) : -1;
+    if (k.size() > 15 && (c < 0 || c >= kClasses))
+      return set_error(GMBE_ERR_ARG, "size class %d out of range", c);
     if ((rc = range(2, 8))) return rc;
     // keep the LDS arena if that variant is built, else take the one that is
-    int lds = kn.lds_kb;
+    int lds = c < 0 ? kn.lds_kb : class_lds(b, c);
     if (!pick_kernel(lds, (int)value, false).fn) lds = 7;
-    return set_variant(b, (int)value, lds);
+    return set_variant(b, (int)value, lds, c);
   }
-  if (k == "lds_kb") {
+  if (k == "lds_kb" || k.rfind("lds_kb.", 0) == 0) {
+    const int c = k.size() > 6 ? atoi(k.c_str() + 7) : -1;
+    if (k.size() > 6 && (c < 0 || c >= kClasses))
+      return set_error(GMBE_ERR_ARG, "size class %d out of range", c);
     if ((rc = range(0, 24))) return rc;
-    return set_variant(b, kn.waves, (int)value);
+    return set_variant(b, c < 0 ? kn.waves : class_waves(b, c), (int)value, c);
   }
   int64_t* slot = nullptr;
   int islot = -1;
@@ -388,6 +405,12 @@ static int gmbe_batch_get_impl(const gmbe_batch* b, const char* key, int64_t* va
   const std::string k(key);
   if (k == "waves_per_block") *value = b->lane.knobs.waves;
   else if (k == "lds_kb") *value = b->lane.knobs.lds_kb;
+  else if (k.rfind("waves_per_block.", 0) == 0 || k.rfind("lds_kb.", 0) == 0) {
+    const bool w = k[0] == 'w';
+    const int c = atoi(k.c_str() + (w ? 16 : 7));
+    if (c < 0 || c >= kClasses) return set_error(GMBE_ERR_ARG, "size class %d out of range", c);
+    *value = w ? class_waves(b, c) : class_lds(b, c);
+  }
   else if (k == "n_classes") *value = kClasses;
   else if (k == "kernel_time_us") *value = (int64_t)(b->last_kernel_s * 1e6);
   else if (k == "wall_time_us") *value = (int64_t)(b->last_wall_s * 1e6);

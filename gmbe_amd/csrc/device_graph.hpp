@@ -69,8 +69,10 @@ hipError_t launch_reset_ctx(hipStream_t st, const mbe::Params& P, unsigned long 
 hipError_t launch_init_left(hipStream_t st, const mbe::Params& P, unsigned long long n);
 hipError_t launch_fill(hipStream_t st, unsigned long long* p, unsigned long long n,
                        unsigned long long v);
-// Control block for a new launch: zero, then work_left = n_roots.
-hipError_t launch_ctl_start(hipStream_t st, unsigned long long* ctl, unsigned long long n_roots);
+// Control block for a new launch: zero, then work_left = n_roots; with
+// keep_deadline the run's deadline word stays (a later launch of one run).
+hipError_t launch_ctl_start(hipStream_t st, unsigned long long* ctl, unsigned long long n_roots,
+                            bool keep_deadline = false);
 // Control block for the wide pass over the tasks the main pass deferred:
 // C_WIDE_N = min(n_deferred, cap), work_left = C_WIDE_N, the list cursor and
 // the deadline kept, the rest zero.

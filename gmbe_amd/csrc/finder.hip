@@ -51,9 +51,11 @@ __global__ void fill_u64(unsigned long long* p, unsigned long long n, unsigned l
 
 // Control block of a launch: everything zero but work_left.
 __global__ void ctl_start(unsigned long long* ctl, unsigned long long words, unsigned long long wl,
-                          unsigned long long n_roots) {
+                          unsigned long long n_roots, unsigned long long keep) {
+  const unsigned long long kept = keep < words ? ctl[keep] : 0ull;
+  __syncthreads();
   for (unsigned long long i = threadIdx.x; i < words; i += blockDim.x)
-    ctl[i] = i == wl ? n_roots : 0ull;
+    ctl[i] = i == wl ? n_roots : i == keep ? kept : 0ull;
 }
 
 // Control block of the wide pass, from the deferred count the main pass left.
@@ -139,9 +141,11 @@ hipError_t launch_fill(hipStream_t st, unsigned long long* p, unsigned long long
   hipLaunchKernelGGL(mbe::fill_u64, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
   return hipGetLastError();
 }
-hipError_t launch_ctl_start(hipStream_t st, unsigned long long* ctl, unsigned long long n_roots) {
+hipError_t launch_ctl_start(hipStream_t st, unsigned long long* ctl, unsigned long long n_roots,
+                            bool keep_deadline) {
   hipLaunchKernelGGL(mbe::ctl_start, dim3(1), dim3(256), 0, st, ctl, kCtlWords,
-                     (unsigned long long)C_WORK_LEFT * kCtlStride, n_roots);
+                     (unsigned long long)C_WORK_LEFT * kCtlStride, n_roots,
+                     keep_deadline ? (unsigned long long)C_DEADLINE * kCtlStride : ~0ull);
   return hipGetLastError();
 }
 hipError_t launch_ctl_wide(hipStream_t st, unsigned long long* ctl, unsigned long long cap) {

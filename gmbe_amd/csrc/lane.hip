@@ -93,6 +93,42 @@ void LanePack::finish(bool light_first) {
   for (uint32_t j = 0; j < n; ++j) root_base[j + 1] = root_base[j] + table[order[j]].nV;
 }
 
+void LanePack::finish_classes(const std::vector<int>& cls, int n_classes) {
+  const uint32_t n = (uint32_t)table.size();
+  order.resize(n);
+  for (uint32_t i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    return cls[a] != cls[b] ? cls[a] < cls[b] : cost[a] > cost[b];
+  });
+  seg_first.clear();
+  seg_class.clear();
+  for (uint32_t j = 0; j < n; ++j)
+    if (j == 0 || cls[order[j]] != cls[order[j - 1]]) {
+      seg_first.push_back(j);
+      seg_class.push_back(cls[order[j]] < n_classes ? cls[order[j]] : n_classes - 1);
+    }
+  seg_first.push_back(n);
+  // one root_base segment per class: its n_c + 1 prefix sums, from 0
+  root_base.clear();
+  for (size_t s = 0; s + 1 < seg_first.size(); ++s) {
+    uint64_t acc = 0;
+    root_base.push_back(0ull);
+    for (uint32_t j = seg_first[s]; j < seg_first[s + 1]; ++j) {
+      acc += table[order[j]].nV;
+      root_base.push_back(acc);
+    }
+  }
+  if (n == 0) root_base.assign(1, 0ull);
+}
+
+uint64_t LanePack::total_roots() const {
+  if (root_base.empty()) return 0;
+  if (seg_first.empty()) return root_base.back();
+  uint64_t t = 0;
+  for (size_t s = 0; s + 1 < seg_first.size(); ++s) t += root_base[seg_first[s + 1] + s];
+  return t;
+}
+
 void LanePack::clear() {
   voff.clear();
   uoff.clear();
@@ -104,6 +140,8 @@ void LanePack::clear() {
   cost.clear();
   order.clear();
   root_base.clear();
+  seg_first.clear();
+  seg_class.clear();
   max_nV = max_dmaxV = 0;
 }
 
@@ -126,7 +164,7 @@ int Lane::alloc(Engine* e, const LaneCaps& c, const LaneKnobs& k) {
   LANE_TRY(hipMalloc(&kU, std::max<uint64_t>(c.max_uoff, 1) * 8));
   LANE_TRY(hipMalloc(&kV, std::max<uint64_t>(c.max_voff, 1) * 8));
   LANE_TRY(hipMalloc(&gt, ng * sizeof(mbe::DevGraph)));
-  LANE_TRY(hipMalloc(&root_base, (ng + 1) * 8));
+  LANE_TRY(hipMalloc(&root_base, (ng + kLaneClasses) * 8));
   LANE_TRY(hipMalloc(&order, ng * 4));
   LANE_TRY(hipMalloc(&per_graph, ng * kPerGraph * 8));
   LANE_TRY(hipMalloc(&ctl, kCtlWords * 8));
@@ -135,11 +173,19 @@ int Lane::alloc(Engine* e, const LaneCaps& c, const LaneKnobs& k) {
   // the arrays of a full lane plus the 16-byte alignment of each of the nine
   // pieces an upload stages
   stage_bytes = (c.max_voff + c.max_uoff) * 16 + c.max_edges * 8 +
-                ng * (sizeof(mbe::DevGraph) + 8 + 4) + 8 + 9 * 16;
+                ng * (sizeof(mbe::DevGraph) + 8 + 4) + 8 * kLaneClasses + 9 * 16;
   LANE_TRY(hipHostMalloc((void**)&stage, stage_bytes, hipHostMallocDefault));
   LANE_TRY(hipHostMalloc((void**)&out_host, (ng * kPerGraph + mbe::A_COUNT) * 8,
                          hipHostMallocDefault));
   return alloc_run();
+}
+
+KernelChoice Lane::class_kernel(int c, int* blocks) const {
+  const int w = c >= 0 && c < kLaneClasses && cls_waves[c] > 0 ? cls_waves[c] : knobs.waves;
+  const int l = c >= 0 && c < kLaneClasses && cls_lds[c] >= 0 ? cls_lds[c] : knobs.lds_kb;
+  const KernelChoice k = pick_kernel(l, w, false);
+  if (blocks) *blocks = k.fn ? (knobs.blocks > 0 ? knobs.blocks : resident_blocks(k, eng->cu_count)) : 0;
+  return k;
 }
 
 int Lane::alloc_run() {
@@ -151,7 +197,16 @@ int Lane::alloc_run() {
   const int blocks = knobs.blocks > 0 ? knobs.blocks : resident_blocks(kc, eng->cu_count);
   // the wide pass runs on at most as many waves as the main pass
   wblocks = std::max(1, std::min(resident_blocks(wk, eng->cu_count), blocks * kc.waves / wk.waves));
-  const int nw = blocks * kc.waves;
+  // slabs for the largest grid of any class's kernel
+  int nw = blocks * kc.waves;
+  for (int c = 0; c < kLaneClasses; ++c) {
+    int cb = 0;
+    const KernelChoice k = class_kernel(c, &cb);
+    if (!k.fn)
+      return set_error(GMBE_ERR_ARG, "no kernel for class %d (waves per block %d, lds_kb %d)", c,
+                       cls_waves[c], cls_lds[c]);
+    nw = std::max(nw, cb * k.waves);
+  }
   mbe::Scratch lay = scratch_layout(std::max<uint32_t>(caps.max_nV, 1),
                                     std::max<uint32_t>(caps.max_dmaxV, 1),
                                     (uint64_t)knobs.arena_kb * 1024ull,
@@ -238,6 +293,16 @@ int Lane::upload(const LanePack& p) {
   LANE_TRY(put(p.order.data(), p.order.size() * 4, order));
   n_graphs = n;
   n_roots = p.total_roots();
+  segs.clear();
+  for (size_t s = 0; s + 1 < p.seg_first.size(); ++s) {
+    LaneSeg g;
+    g.g0 = p.seg_first[s];
+    g.n = p.seg_first[s + 1] - p.seg_first[s];
+    g.rb_off = g.g0 + s;
+    g.roots = p.root_base[g.rb_off + g.n];
+    g.cls = p.seg_class[s];
+    segs.push_back(g);
+  }
   return GMBE_OK;
 }
 
@@ -321,6 +386,7 @@ int Lane::upload_many(const std::vector<const LanePack*>& parts, bool light_firs
   LANE_TRY(h2d(order, s_ord, (uint64_t)n * 4));
   n_graphs = n;
   n_roots = rb[n];
+  segs.clear();
   return GMBE_OK;
 }
 
@@ -377,17 +443,43 @@ mbe::Params Lane::params(bool wide) const {
 int Lane::record_run() {
   const mbe::Params P = params(false), Pw = params(true);
   const uint64_t ng = std::max<uint32_t>(n_graphs, 1);
-  LANE_TRY(launch_ctl_start(st, ctl, n_roots));
   LANE_TRY(launch_fill(st, acc, mbe::A_COUNT, 0ull));
   LANE_TRY(launch_fill(st, per_graph, ng * kPerGraph, 0ull));
   if (n_graphs > 0) LANE_TRY(launch_init_left(st, P, n_graphs));
-  LANE_TRY(launch_init_queue(st, qseq, qcap));
-  LANE_TRY(launch_reset_ctx(st, P, (unsigned long long)nwaves * layout.ctx_slots));
-  if (n_roots > 0) LANE_TRY(launch_persistent(kc, nblocks, st, P));
-  // wide pass over the deferred tasks; its grid exits at once when there are none
-  LANE_TRY(launch_ctl_wide(st, ctl, knobs.deferred_cap));
-  LANE_TRY(launch_init_queue(st, qseq, qcap));
-  if (n_roots > 0) LANE_TRY(launch_persistent(wk, wblocks, st, Pw));
+  if (segs.empty()) {
+    LANE_TRY(launch_ctl_start(st, ctl, n_roots));
+    LANE_TRY(launch_init_queue(st, qseq, qcap));
+    LANE_TRY(launch_reset_ctx(st, P, (unsigned long long)nwaves * layout.ctx_slots));
+    if (n_roots > 0) LANE_TRY(launch_persistent(kc, nblocks, st, P));
+    // wide pass over the deferred tasks; its grid exits at once when there are none
+    LANE_TRY(launch_ctl_wide(st, ctl, knobs.deferred_cap));
+    LANE_TRY(launch_init_queue(st, qseq, qcap));
+    if (n_roots > 0) LANE_TRY(launch_persistent(wk, wblocks, st, Pw));
+  } else {
+    // one main and one wide launch per size class, each over the class's
+    // segment of the claim order with the class's kernel variant
+    bool first = true;
+    for (const LaneSeg& g : segs) {
+      if (g.roots == 0) continue;
+      int cb = 0;
+      const KernelChoice ck = class_kernel(g.cls, &cb);
+      mbe::Params Pc = P;
+      Pc.s.n_roots = g.roots;
+      Pc.s.root_base = root_base + g.rb_off;
+      Pc.s.order = order + g.g0;
+      Pc.s.n_graphs = g.n;
+      if (knobs.share_queue < 0) Pc.share_queue = cb * ck.waves;
+      const int cwb = std::max(1, std::min(resident_blocks(wk, eng->cu_count), cb * ck.waves / wk.waves));
+      LANE_TRY(launch_ctl_start(st, ctl, g.roots, !first));  // one deadline for the run
+      first = false;
+      LANE_TRY(launch_init_queue(st, qseq, qcap));
+      LANE_TRY(launch_reset_ctx(st, Pc, (unsigned long long)nwaves * layout.ctx_slots));
+      LANE_TRY(launch_persistent(ck, cb, st, Pc));
+      LANE_TRY(launch_ctl_wide(st, ctl, knobs.deferred_cap));
+      LANE_TRY(launch_init_queue(st, qseq, qcap));
+      LANE_TRY(launch_persistent(wk, cwb, st, Pw));
+    }
+  }
   LANE_TRY(hipMemcpyAsync(out_host, per_graph, ng * kPerGraph * 8, hipMemcpyDeviceToHost, st));
   LANE_TRY(hipMemcpyAsync(out_host + ng * kPerGraph, acc, mbe::A_COUNT * 8,
                           hipMemcpyDeviceToHost, st));

@@ -51,6 +51,10 @@ struct LanePack {
   std::vector<double> cost;          // claim-order cost proxy per graph
   std::vector<uint32_t> order;       // claim order of the graphs
   std::vector<unsigned long long> root_base;
+  // with size classes: order position of each class's first graph (plus the
+  // end), and root_base holds one segment per class (n_c + 1 entries from 0)
+  std::vector<uint32_t> seg_first;
+  std::vector<int> seg_class;
   uint32_t max_nV = 0, max_dmaxV = 0;
   // add graph g with split thresholds (bh, bs); returns its index
   uint32_t add(const gmbe_graph& g, int bh, int bs);
@@ -59,9 +63,23 @@ struct LanePack {
   // claim order by a cost proxy (sum of squared U degrees): heaviest first
   // for throughput, lightest first when requests race a deadline
   void finish(bool light_first = false);
+  // claim order by size class (cls[i] of graph i, in [0, n_classes)), then
+  // by cost, heaviest first; the kernel of each class runs over its own
+  // segment of the order
+  void finish_classes(const std::vector<int>& cls, int n_classes);
   void clear();
-  uint64_t total_roots() const { return root_base.empty() ? 0 : root_base.back(); }
+  uint64_t total_roots() const;
 };
+
+// Claim range of one size class in a lane (finish_classes).
+struct LaneSeg {
+  uint32_t g0 = 0, n = 0;  // first order position, graphs
+  uint64_t rb_off = 0;     // its root_base segment
+  uint64_t roots = 0;
+  int cls = 0;
+};
+
+constexpr int kLaneClasses = 4;
 
 struct Lane {
   Engine* eng = nullptr;
@@ -95,11 +113,17 @@ struct Lane {
   // current contents
   uint32_t n_graphs = 0;
   uint64_t n_roots = 0;
+  std::vector<LaneSeg> segs;  // per-class claim ranges (empty: one range)
   KernelChoice kc{}, wk{};
+  // per-class kernel variants (used when the contents have class segments):
+  // waves per block and LDS KB per wave; -1 = the lane's knobs
+  int cls_waves[kLaneClasses] = {-1, -1, -1, -1}, cls_lds[kLaneClasses] = {-1, -1, -1, -1};
 
   int alloc(Engine* e, const LaneCaps& c, const LaneKnobs& k);
   // (re)size the run buffers for the current knobs (after a knob change)
   int alloc_run();
+  // kernel variant of class c (its own, else the knobs'), its block count
+  KernelChoice class_kernel(int c, int* blocks) const;
   void release();
   // Copy a pack into the lane (async on st; the pack must stay alive until
   // the stream has passed the copy).

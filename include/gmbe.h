@@ -271,30 +271,33 @@ void gmbe_finder_destroy(gmbe_finder* f);
 /* ---- batched form ------------------------------------------------------ */
 
 /* Many independent graphs (working form, e.g. from gmbe_graph_derive with
- * -t 1 -o 1 -f) enumerated by one persistent launch over a concatenated
- * device CSR with a per-graph table.  The launch sequence is captured once in
- * a HIP graph and replayed by every gmbe_batch_execute.  Graphs fall in four
- * size classes by edge count ([0,2k), [2k,8k), [8k,32k), [32k,inf)); each
- * class has its own split thresholds.  No reference counterpart (the
- * reference runs one graph per process, src/main.cu:126-147).  The graphs
- * are borrowed and must outlive the batch. */
+ * -t 1 -o 1 -f) enumerated over a concatenated device CSR with a per-graph
+ * table.  Graphs fall in four size classes by edge count ([0,2k), [2k,8k),
+ * [8k,32k), [32k,inf)); each class has its own kernel variant and split
+ * thresholds, and a run is one persistent launch pair (main and wide pass)
+ * per class over that class's graphs.  The run is captured once in a HIP
+ * graph and replayed by every gmbe_batch_execute.  No reference counterpart
+ * (the reference runs one graph per process, src/main.cu:126-147).  The
+ * graphs are borrowed and must outlive the batch. */
 int gmbe_batch_create(const gmbe_graph* const* graphs, int n, int bound_height,
                       int bound_size, gmbe_batch** out);
-/* Knobs: "waves_per_block" (2, 4, 8; built with lds_kb 16, any, 4
- * respectively: setting it moves lds_kb to a built value), "lds_kb" (LDS
- * list arena per wave, KB),
- * "bound_height.<c>", "bound_size.<c>" (class c), "prune", "max_prefix",
- * "group", "share", "split", "share_work", "merge_min", "merge_branch",
- * "late_share", "arena_kb", "ctx_kb", "ctx_slots";
+/* Knobs: "waves_per_block" (2, 4, 8; built with lds_kb 7, any of 0/3/7/16,
+ * and 7 respectively: setting it moves lds_kb to a built value), "lds_kb"
+ * (LDS list arena per wave, KB) -- for every class, or for class c with
+ * "waves_per_block.<c>", "lds_kb.<c>"; "bound_height.<c>", "bound_size.<c>"
+ * (class c), "prune", "max_prefix", "group", "share", "split", "share_work",
+ * "merge_min", "merge_branch", "late_share", "arena_kb", "ctx_kb",
+ * "ctx_slots", "dom_k", "dom_skip", "recompute", "count_bytes";
  * "per_graph_launch" = 1 runs the graphs one launch (and one host wait) at a
- * time with the same kernel, the looped baseline;
- * "autotune" = 1 measures the kernel variants and then each class's
- * threshold candidates on a stratified sample of this batch (8 graphs per
- * class) and keeps the fastest (synchronous); "verbose" = 1 makes it print
- * one line per trial on stderr. */
+ * time with their class's kernel, the looped baseline;
+ * "autotune" = 1 measures, for each class on a sample of its graphs (8 at
+ * most), the kernel variants and then the threshold candidates, and keeps
+ * the fastest (synchronous); "verbose" = 1 makes it print one line per trial
+ * on stderr. */
 int gmbe_batch_set(gmbe_batch* b, const char* key, int64_t value);
 /* Read back a knob or a choice of the autotuner: "waves_per_block",
- * "lds_kb", "n_classes", "bound_height.<c>", "bound_size.<c>", "class.<i>"
+ * "lds_kb" (the defaults), "waves_per_block.<c>", "lds_kb.<c>",
+ * "n_classes", "bound_height.<c>", "bound_size.<c>", "class.<i>"
  * (size class of graph i), "kernel_time_us", "wall_time_us" (last execute). */
 int gmbe_batch_get(const gmbe_batch* b, const char* key, int64_t* value);
 /* per_graph may be NULL; otherwise it receives n results in input order

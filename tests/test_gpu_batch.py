@@ -101,13 +101,41 @@ def test_autotuner_keeps_results(gpu, small_set, expected):
     gs = [small_set[i] for i in pick]
     b = Batch(gs)
     b.set("autotune", 1)
-    assert b.get("waves_per_block") in (2, 4, 8)
-    assert b.get("lds_kb") in (0, 3, 7, 16)
     for c in range(b.get("n_classes")):
+        assert b.get(f"waves_per_block.{c}") in (2, 4, 8)
+        assert b.get(f"lds_kb.{c}") in (0, 3, 7, 16)
         assert b.get(f"bound_height.{c}") >= 0
     per, _ = b.execute()
     assert [r.digest() for r in per] == [expected[i] for i in pick]
     b.close()
+
+
+def test_per_class_kernel_variants(gpu, small_set, expected):
+    """Each size class with its own kernel variant (one launch pair per class
+    in the captured run): the per-graph results stay those of the plain
+    finder, in the captured run and in the looped one."""
+    b = Batch(small_set)
+    variants = [(2, 7), (8, 7), (4, 0), (4, 16)]
+    for c in range(b.get("n_classes")):
+        w, l = variants[c % len(variants)]
+        b.set(f"waves_per_block.{c}", w)
+        b.set(f"lds_kb.{c}", l)
+        assert (b.get(f"waves_per_block.{c}"), b.get(f"lds_kb.{c}")) == (w, l)
+    per, _ = b.execute()
+    assert [r.digest() for r in per] == expected
+    per, _ = b.execute()  # a replay of the captured run
+    assert [r.digest() for r in per] == expected
+    light = _light(small_set)
+    lb = Batch([small_set[i] for i in light])
+    for c in range(lb.get("n_classes")):
+        w, l = variants[c % len(variants)]
+        lb.set(f"waves_per_block.{c}", w)
+        lb.set(f"lds_kb.{c}", l)
+    lb.set("per_graph_launch", 1)
+    per, _ = lb.execute()
+    assert [r.digest() for r in per] == [expected[i] for i in light]
+    b.close()
+    lb.close()
 
 
 def test_bad_knobs_are_errors(gpu, small_set):
