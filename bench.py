@@ -436,9 +436,9 @@ def main():
             sw = f.stats()
             mn, mb, ml = sw["main_nodes"], sw["main_alg_bytes"], sw["main_list_bytes"]
             wn = max(1, r.nodes - mn)
-            model = {"main": {"list": ml / max(1, mn), "survey": mb / max(1, mn)},
+            model = {"main": {"list": ml / max(1, mn), "id_list": mb / max(1, mn)},
                      "wide": {"list": (sw["list_bytes"] - ml) / wn,
-                              "survey": (r.alg_bytes - mb) / wn},
+                              "id_list": (r.alg_bytes - mb) / wn},
                      "sample": f"processing ranks 0 mod {4 * K}: {r.nodes} nodes, "
                                f"{sw['list_bytes']} list bytes, {r.alg_bytes} bytes of the "
                                "id-list model"}
@@ -490,7 +490,7 @@ def main():
         n = max(1, kd["launches"])
         kd["avg_launch_s"] = kd["s"] / n
         kd["nodes_per_launch"] = kd["nodes"] / n
-        for b in ("list", "survey"):
+        for b in ("list", "id_list"):
             bpn = model[name][b] if model else None
             per_launch = bpn * kd["nodes_per_launch"] if bpn is not None else None
             kd[f"{b}_bytes_per_node"] = bpn
