@@ -52,6 +52,26 @@ uint64_t predict_bytes(const gmbe_graph& g) {
          8ull * mbe::kPerGraph;
 }
 
+// List memory a request surely needs in a wave's arena: the root list of a
+// task holds one entry of 16 + 8 W bytes (W = bitset words of the root's L,
+// at most 16) per 2-hop candidate, and a root v has at least deg(u) - 1 of
+// them for each neighbour u.  The largest such lower bound over the roots;
+// a request above the arena would overflow it.  O(E).
+uint64_t predict_stack_bytes(const gmbe_graph& g) {
+  uint64_t best = 0;
+  for (uint32_t v = 0; v < g.nV; ++v) {
+    uint64_t cand = 0;
+    for (int64_t e = g.v_off[v]; e < g.v_off[v + 1]; ++e) {
+      const uint32_t u = g.v_adj[e];
+      cand = std::max<uint64_t>(cand, (uint64_t)(g.u_off[u + 1] - g.u_off[u]) - 1);
+    }
+    const uint64_t deg = (uint64_t)(g.v_off[v + 1] - g.v_off[v]);
+    const uint64_t words = std::min<uint64_t>((deg + 63) / 64, (uint64_t)mbe::kMaxW);
+    best = std::max<uint64_t>(best, cand * (16 + 8 * words));
+  }
+  return best;
+}
+
 }  // namespace
 
 struct gmbe_server {
@@ -70,6 +90,7 @@ struct gmbe_server {
   std::deque<gmbe_serve_done> done;
   uint64_t next_ticket = 1;
   int64_t request_ms = 0;  // deadline of each launch (0 = none)
+  std::atomic<int64_t> stack_limit_kb{0};  // admission: predicted list memory per wave
   // statistics (written by the server thread under mu)
   uint64_t launches = 0, launched = 0;
   double host_s = 0, gpu_s = 0;
@@ -209,6 +230,16 @@ void server_loop(gmbe_server* s) {
 int submit(gmbe_server* s, gmbe_graph* w, uint64_t* ticket) {
   Request r;
   r.bytes = predict_bytes(*w);
+  // admission by predicted list memory: a request whose root lists cannot fit
+  // a wave's arena would overflow it; refuse it before it is queued
+  const uint64_t stack = predict_stack_bytes(*w);
+  const uint64_t limit = (uint64_t)s->stack_limit_kb.load() * 1024ull;
+  if (stack > limit) {
+    gmbe_graph_destroy(w);
+    return set_error(GMBE_ERR_REJECTED,
+                     "request needs %llu bytes of list memory per wave (predicted); the limit is "
+                     "%llu", (unsigned long long)stack, (unsigned long long)limit);
+  }
   r.t_submit = now_s();
   r.nU = w->nU;
   r.nV = w->nV;
@@ -279,6 +310,7 @@ static int gmbe_serve_create_impl(const gmbe_graph* resident, int slots, uint64_
   kn.ctx_kb = 512;
   kn.queue_log2 = 18;
   kn.deferred_cap = 1ull << 16;
+  s->stack_limit_kb = kn.arena_kb;
   for (int l = 0; l < kLanes && rc == GMBE_OK; ++l) {
     rc = s->lanes[l].alloc(&s->eng, caps, kn);
     if (rc == GMBE_OK) {
@@ -311,6 +343,15 @@ int gmbe_serve_create(const gmbe_graph* resident, int slots, uint64_t mem_budget
 static int gmbe_serve_set_impl(gmbe_server* s, const char* key, int64_t value) {
   if (!s || !key) return set_error(GMBE_ERR_ARG, "null argument");
   const std::string k(key);
+  if (k == "stack_limit_kb") {
+    // at most the lanes' arena: a larger limit would admit requests that overflow
+    const int64_t arena = s->lanes[0].knobs.arena_kb;
+    if (value < 1 || value > arena)
+      return set_error(GMBE_ERR_ARG, "stack_limit_kb=%lld out of range [1, %lld]", (long long)value,
+                       (long long)arena);
+    s->stack_limit_kb = value;
+    return GMBE_OK;
+  }
   if (k == "request_ms") {
     if (value < 0 || value > (1ll << 32))
       return set_error(GMBE_ERR_ARG, "request_ms=%lld out of range", (long long)value);

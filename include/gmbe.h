@@ -327,9 +327,13 @@ typedef struct {
 int gmbe_serve_create(const gmbe_graph* resident, int slots,
                       uint64_t mem_budget, int bound_height, int bound_size,
                       gmbe_server** out);
-/* Knob: "request_ms" (default 0 = none): deadline of each launch; requests
+/* Knobs: "request_ms" (default 0 = none): deadline of each launch; requests
  * of a launch that reaches it finish with status GMBE_ERR_DEADLINE and
- * partial counts, the others with their exact result. */
+ * partial counts, the others with their exact result.  "stack_limit_kb"
+ * (default and maximum: the lanes' per-wave list arena, 4096): a request is
+ * admitted only if the list memory its largest task root surely needs (one
+ * entry per 2-hop candidate, counted from its neighbours' degrees) fits;
+ * otherwise submit returns GMBE_ERR_REJECTED. */
 int gmbe_serve_set(gmbe_server* s, const char* key, int64_t value);
 /* Submit a stand-alone graph (file orientation; copied and brought to the
  * -t 1 -o 1 -f working form on the calling thread). */

@@ -5,8 +5,9 @@ resident graph is the full-size power-law graph of the served workload."""
 import numpy as np
 import pytest
 
+import graphs
 import oracle
-from gmbe_amd import GmbeError, Server, enumerate_graph, workloads
+from gmbe_amd import BiGraph, GmbeError, Server, enumerate_graph, workloads
 
 pytestmark = pytest.mark.gpu
 RUN = workloads.RUN_OPTION
@@ -96,6 +97,28 @@ def test_zero_budget_rejects(gpu):
     with pytest.raises(GmbeError) as e:
         s.submit_graph(workloads.small_graph(workloads.serve_small_seed(1)))
     assert e.value.code == -7
+    s.close()
+
+
+def test_list_memory_admission(gpu):
+    """A request whose task roots surely need more list memory per wave than
+    stack_limit_kb allows is refused at submit; one that fits is served
+    exactly.  Crown graph S_60: every root has at least 58 candidates of 24
+    bytes (1,392 B > 1 KB); a matching has none."""
+    s = Server(None, slots=4, mem_budget=1 << 24)
+    s.set("stack_limit_kb", 1)
+    crown = BiGraph.from_rows(*graphs.crown(60)[:2])
+    with pytest.raises(GmbeError) as e:
+        s.submit_graph(crown)
+    assert e.value.code == -7 and "list memory" in str(e.value)
+    with pytest.raises(GmbeError):
+        s.set("stack_limit_kb", 1 << 30)  # above the lanes' arena
+    match = BiGraph.from_rows(*graphs.matching(7)[:2])
+    t = s.submit_graph(match)
+    done = {}
+    while t not in done:
+        done.update({d.ticket: d for d in s.poll(wait=True)})
+    assert done[t].status == 0 and done[t].maximal == 7
     s.close()
 
 
