@@ -138,6 +138,19 @@ def test_per_class_kernel_variants(gpu, small_set, expected):
     lb.close()
 
 
+def test_light_graphs_of_the_first_1024_match_the_oracle(gpu):
+    """Config 4's first 1,024 graphs, all those under 16,000 edges (778 of
+    them, 8.3 x 10^7 bicliques): one batch, every graph's count and digest
+    equal to the oracle's."""
+    gs = [g for g in workloads.batch_graphs(1024) if g.stats()["edge_size"] < 16000]
+    assert len(gs) > 700
+    b = Batch(gs)
+    per, _ = b.execute()
+    b.close()
+    for i, (g, r) in enumerate(zip(gs, per)):
+        assert r.digest() == oracle.gmbe(g.oracle_graph(), threads=8).digest(), i
+
+
 def test_bad_knobs_are_errors(gpu, small_set):
     b = Batch(small_set[:2])
     with pytest.raises(GmbeError):
