@@ -97,6 +97,22 @@ def test_unilateral_order_is_a_permutation(tmp_path):
     assert [src[o] for o in v_orig] == rows
 
 
+def test_unilateral_order_known_answer(tmp_path):
+    """Order 3 on a graph whose 2-hop graph is a star (centre V0, leaves
+    V1-V3) beside a triangle (V4-V6), each 2-hop edge one U vertex of degree
+    two.  Worked by hand through the reference's bucket peeling
+    (src/BiGraph.cu:93-210): 2-hop degrees 3,1,1,1,2,2,2 give the initial
+    order V1 V2 V3 | V4 V5 V6 | V0.  Taking V1 lowers V0 to 2 (it already sits
+    first in its new bucket); taking V2 lowers V0 to 1, which swaps it with
+    V4, the first of bucket 2.  The order is V1 V2 V3 V0 V5 V6 V4: a plain
+    sort by 2-hop degree would keep V4 before V5 and V6."""
+    rows = ["0 1 2", "0", "1", "2", "3 4", "3 5", "4 5"]
+    g = BiGraph.read_adj(write(tmp_path, "\n".join(rows) + "\n"), uvtrans=0, order=3,
+                         fast_mode=0)
+    _, _, _, v_orig = rows_of(g)
+    assert list(v_orig) == [1, 2, 3, 0, 5, 6, 4]
+
+
 def test_derive_keeps_original_ids(tmp_path):
     text = "0 1 2\n3\n1 2\n0 2\n2 3 4\n"
     raw = BiGraph.read_adj(write(tmp_path, text), order=0, fast_mode=1)
