@@ -46,9 +46,11 @@
  *
  * Sampling and time limits: every enumerator takes the task roots of
  * processing ranks phase, phase + stride, ... (phase may exceed the stride).
- * time_limit stops claiming roots once passed; orc_mbea also abandons the
- * subtrees it is in, so its counts are then partial but its rate is that of
- * the work done (the benchmark's CPU baseline reads it that way).
+ * time_limit stops claiming roots once passed; orc_mbea, orc_imbea and
+ * orc_pmbe also abandon the subtrees they are in, so their counts are then
+ * partial but their rate is that of the work done (the benchmark's CPU
+ * baseline reads it that way), and orc_result.cut counts the threads so cut:
+ * a sample is complete only when every root was claimed and cut is 0.
  *
  * Parity pinning of each function: see tests/test_oracle.py (closed forms,
  * brute force, the three-way agreement, and the counts of the reference's
